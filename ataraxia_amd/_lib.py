@@ -1,0 +1,143 @@
+"""ctypes binding of libatx.so (the C ABI declared in include/atx.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C ataraxia_amd/csrc``).  Nothing here falls back to a CPU path: if
+the library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libatx.so")
+
+
+class AtxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"atx error {code}: {msg}")
+        self.code = code
+
+
+ATX_OK = 0
+ATX_ERR_INVALID = -1
+ATX_ERR_HIP = -2
+ATX_ERR_NOMEM = -3
+ATX_ERR_STATE = -4
+ATX_ERR_IO = -5
+ATX_ERR_BUDGET = -6
+ATX_ERR_NOTFOUND = -7
+ATX_ERR_NODEVICE = -8
+
+ATX_ACCUM_NAIVE = 0
+ATX_ACCUM_KAHAN = 1
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("albedo", C.c_float * 3), ("roughness", C.c_float), ("metallic", C.c_float),
+                ("F0", C.c_float * 3), ("emission_color", C.c_float * 3),
+                ("emission_intensity", C.c_float), ("id", C.c_int32)]
+
+
+class Light(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("accumulation", C.c_uint8), ("sky_light", C.c_uint8), ("reserved", C.c_uint8 * 2),
+                ("max_bounces", C.c_int32)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("fov_deg", C.c_float),
+                ("near_clip", C.c_float), ("far_clip", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("sphere_tests", C.c_uint64),
+                ("exact_retraces", C.c_uint64), ("gpu_ms", C.c_double)]
+
+
+assert C.sizeof(Sphere) == 20 and C.sizeof(Material) == 52 and C.sizeof(Light) == 28
+assert C.sizeof(Settings) == 8
+
+# numpy views of the same layouts (packed, little endian)
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("material", "<i4")])
+MATERIAL_DTYPE = np.dtype([("albedo", "<f4", 3), ("roughness", "<f4"), ("metallic", "<f4"),
+                           ("F0", "<f4", 3), ("emission_color", "<f4", 3),
+                           ("emission_intensity", "<f4"), ("id", "<i4")])
+LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("color", "<f4", 3), ("intensity", "<f4")])
+assert SPHERE_DTYPE.itemsize == 20 and MATERIAL_DTYPE.itemsize == 52 and LIGHT_DTYPE.itemsize == 28
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_I = C.c_int
+
+# name -> (restype, argtypes); every name here is declared in include/atx.h
+PROTOTYPES = {
+    "atx_last_error": (C.c_char_p, []),
+    "atx_version": (C.c_char_p, []),
+    "atx_device_count": (_I, [C.POINTER(_I)]),
+    "atx_device_synchronize": (_I, []),
+    "atx_renderer_create": (_I, [C.POINTER(_P)]),
+    "atx_renderer_destroy": (_I, [_P]),
+    "atx_renderer_on_resize": (_I, [_P, _U32, _U32]),
+    "atx_renderer_set_scene": (_I, [_P, _P, _U64, _P, _U64, _P, _U64]),
+    "atx_renderer_set_settings": (_I, [_P, C.POINTER(Settings)]),
+    "atx_renderer_get_settings": (_I, [_P, C.POINTER(Settings)]),
+    "atx_renderer_reset_frame_index": (_I, [_P]),
+    "atx_renderer_get_frame_index": (_I, [_P, C.POINTER(_U32)]),
+    "atx_renderer_set_accumulator": (_I, [_P, _I]),
+    "atx_renderer_render": (_I, [_P, C.POINTER(CameraDesc)]),
+    "atx_renderer_render_frames": (_I, [_P, C.POINTER(CameraDesc), _U32]),
+    "atx_renderer_get_image": (_I, [_P, _P, _U64]),
+    "atx_renderer_get_accumulation": (_I, [_P, _P, _U64]),
+    "atx_renderer_get_stats": (_I, [_P, C.POINTER(RenderStats)]),
+    "atx_renderer_render_debug": (_I, [_P, C.POINTER(CameraDesc), _P, _P, _P, _U32]),
+    "atx_camera_ray_directions": (_I, [C.POINTER(CameraDesc), _P, _U64]),
+    "atx_camera_matrices": (_I, [C.POINTER(CameraDesc), _P, _P]),
+}
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C ataraxia_amd/csrc`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    return (lib.atx_last_error() or b"").decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != ATX_OK:
+        raise AtxError(rc, last_error())
+
+
+def ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib.atx_device_count(C.byref(n))
+    return n.value if rc == ATX_OK else 0

@@ -1,0 +1,124 @@
+// host_common.cpp -- error state, scene layout, device queries, camera C ABI.
+#include "host_common.h"
+
+#include <string.h>
+
+#include "camera_host.h"
+
+namespace atx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material* materials,
+                       uint64_t m, const atx_light* lights, uint64_t l, HostSceneLayout* out)
+{
+    ATX_CHECK_ARG(n == 0 || spheres, "spheres is NULL with num_spheres=%llu", (unsigned long long)n);
+    ATX_CHECK_ARG(m == 0 || materials, "materials is NULL with num_materials=%llu", (unsigned long long)m);
+    ATX_CHECK_ARG(l == 0 || lights, "lights is NULL with num_lights=%llu", (unsigned long long)l);
+    ATX_CHECK_ARG(n < (1ull << 31) && m < (1ull << 28) && l < (1ull << 28), "scene too large");
+    if (n > 0 && m == 0) {
+        set_error("scene has %llu spheres but no material", (unsigned long long)n);
+        return ATX_ERR_INVALID;
+    }
+    out->sph.resize(4 * n);
+    out->sph_mat.resize(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const atx_sphere& s = spheres[i];
+        out->sph[4 * i + 0] = s.center[0];
+        out->sph[4 * i + 1] = s.center[1];
+        out->sph[4 * i + 2] = s.center[2];
+        out->sph[4 * i + 3] = s.radius * s.radius;
+        int32_t id = s.material;
+        if ((uint32_t)id >= m) {
+            fprintf(stderr, "Sphere ID out of bounds: %d\n", id);
+            id = 0;
+        }
+        out->sph_mat[i] = id;
+    }
+    out->mat.resize(12 * m);
+    for (uint64_t i = 0; i < m; ++i) {
+        const atx_material& a = materials[i];
+        float* o = &out->mat[12 * i];
+        o[0] = a.albedo[0]; o[1] = a.albedo[1]; o[2] = a.albedo[2]; o[3] = a.roughness;
+        o[4] = a.F0[0]; o[5] = a.F0[1]; o[6] = a.F0[2]; o[7] = a.metallic;
+        o[8] = a.emission_color[0]; o[9] = a.emission_color[1]; o[10] = a.emission_color[2];
+        o[11] = a.emission_intensity;
+    }
+    out->light.resize(8 * l);
+    for (uint64_t i = 0; i < l; ++i) {
+        const atx_light& a = lights[i];
+        float* o = &out->light[8 * i];
+        o[0] = a.position[0]; o[1] = a.position[1]; o[2] = a.position[2]; o[3] = a.intensity;
+        o[4] = a.color[0]; o[5] = a.color[1]; o[6] = a.color[2]; o[7] = 0.0f;
+    }
+    return ATX_OK;
+}
+
+}  // namespace atx
+
+using namespace atx;
+
+extern "C" {
+
+const char* atx_last_error(void) { return g_last_error.c_str(); }
+
+const char* atx_version(void) { return "ataraxia-amd 0.1 (gfx950)"; }
+
+int atx_device_count(int* count)
+{
+    ATX_CHECK_ARG(count, "count is NULL");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *count = 0;
+        set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+        return ATX_ERR_NODEVICE;
+    }
+    *count = c;
+    return ATX_OK;
+}
+
+int atx_device_synchronize(void)
+{
+    ATX_HIP(hipDeviceSynchronize());
+    return ATX_OK;
+}
+
+int atx_camera_matrices(const atx_camera* cam, float* inv_proj, float* inv_view)
+{
+    ATX_CHECK_ARG(cam && inv_proj && inv_view, "NULL argument");
+    ATX_CHECK_ARG(cam->width > 0 && cam->height > 0, "Width or height cannot be zero");
+    camera_inverse_matrices(cam->position, cam->direction, cam->fov_deg, cam->near_clip, cam->far_clip,
+                            cam->width, cam->height, inv_proj, inv_view);
+    return ATX_OK;
+}
+
+int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_floats)
+{
+    ATX_CHECK_ARG(cam && dirs, "NULL argument");
+    ATX_CHECK_ARG(cam->width > 0 && cam->height > 0, "Width or height cannot be zero");
+    ATX_CHECK_ARG(num_floats >= 3ull * cam->width * cam->height, "dirs holds %llu floats, need %llu",
+                  (unsigned long long)num_floats, 3ull * cam->width * cam->height);
+    float ip[16], iv[16];
+    camera_inverse_matrices(cam->position, cam->direction, cam->fov_deg, cam->near_clip, cam->far_clip,
+                            cam->width, cam->height, ip, iv);
+    for (uint32_t y = 0; y < cam->height; ++y)
+        for (uint32_t x = 0; x < cam->width; ++x) {
+            const f3 d = camera_ray(ip, iv, x, y, cam->width, cam->height);
+            float* o = dirs + 3 * ((size_t)x + (size_t)y * cam->width);
+            o[0] = d.x; o[1] = d.y; o[2] = d.z;
+        }
+    return ATX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// host_common.h -- error reporting and device-buffer ownership for libatx.
+//
+// The reference exits the process on any CUDA error (DeviceMemory.h:7-16);
+// libatx instead records a thread-local message and returns a status code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+#include "../../include/atx.h"
+
+namespace atx {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define ATX_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::atx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),         \
+                             __FILE__, __LINE__);                                          \
+            return e_ == hipErrorOutOfMemory ? ATX_ERR_NOMEM : ATX_ERR_HIP;                \
+        }                                                                                  \
+    } while (0)
+
+#define ATX_CHECK_ARG(cond, ...)          \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::atx::set_error(__VA_ARGS__); \
+            return ATX_ERR_INVALID;       \
+        }                                 \
+    } while (0)
+
+// Move-only owner of one device allocation (the reference's CudaBuffer has a
+// defaulted copy constructor, DeviceMemory.h:33, which can double-free).
+class DevBuf {
+public:
+    DevBuf() = default;
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p_ = o.p_; n_ = o.n_;
+            o.p_ = nullptr; o.n_ = 0;
+        }
+        return *this;
+    }
+    // Grows to at least `bytes` (contents are not preserved on growth).
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= n_ && p_) return hipSuccess;
+        release();
+        if (bytes == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p_, bytes);
+        if (e != hipSuccess) { p_ = nullptr; return e; }
+        n_ = bytes;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p_); }
+    void* get() const { return p_; }
+    size_t bytes() const { return n_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Device-layout copy of one flattened scene (see device_types.h).
+struct HostSceneLayout {
+    std::vector<float> sph;       // 4 per sphere: cx, cy, cz, r*r
+    std::vector<int32_t> sph_mat; // material index per sphere (clamped)
+    std::vector<float> mat;       // 12 per material
+    std::vector<float> light;     // 8 per light
+};
+
+// Builds the device layout; ids outside [0, num_materials) are clamped to 0
+// with a message on stderr, as Renderer::allocateDeviceMemory does
+// (Engine/src/Renderer.cu:30-37).  Fails if spheres exist but no material.
+int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material* materials,
+                       uint64_t m, const atx_light* lights, uint64_t l, HostSceneLayout* out);
+
+}  // namespace atx

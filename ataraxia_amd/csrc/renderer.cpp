@@ -1,0 +1,300 @@
+// renderer.cpp -- the single-scene Renderer behind the C ABI.
+//
+// Mirrors Renderer (Engine/include/Renderer.h:18-66, Engine/src/Renderer.cu):
+// the handle owns the device scene, the accumulation buffer(s), the RGBA8
+// image, the settings and the frame index.  Differences by design:
+//  * rays are generated on the device from the camera's inverse matrices,
+//    so there is no per-frame W*H*12 B upload, malloc or free
+//    (Camera.cpp:197-220);
+//  * no per-frame occupancy/device-property queries or full-device syncs;
+//    work is queued on the handle's own stream and `render` waits on it;
+//  * the RGBA8 image stays on the device until atx_renderer_get_image.
+#include <new>
+#include <stdlib.h>
+#include <string.h>
+
+#include "camera_host.h"
+#include "device_types.h"
+#include "host_common.h"
+
+namespace atx {
+
+struct Renderer {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf sph, sph_mat, mat, light;
+    uint32_t n_sph = 0, n_mat = 0, n_light = 0;
+    bool has_scene = false;
+    DevBuf accum, comp, image, counters;
+    uint32_t width = 0, height = 0;
+    atx_settings settings{1, 0, {0, 0}, 15};  // Scene.h:49-56 defaults
+    uint32_t frame_index = 1;
+    int accumulator = ATX_ACCUM_NAIVE;
+    size_t lds_budget = 64 * 1024;
+    bool exact_loop = false;
+    atx_render_stats stats{};
+};
+
+static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t frames, float* dbg_rad,
+                                  int32_t* dbg_hit, int32_t* dbg_shadow, uint32_t dbg_bounces)
+{
+    ATX_CHECK_ARG(cam, "camera is NULL");
+    if (r->width == 0 || r->height == 0) {
+        set_error("renderer has no viewport: call atx_renderer_on_resize first");
+        return ATX_ERR_STATE;
+    }
+    ATX_CHECK_ARG(cam->width == r->width && cam->height == r->height,
+                  "camera viewport %ux%u does not match renderer %ux%u", cam->width, cam->height, r->width,
+                  r->height);
+    if (!r->has_scene) {
+        set_error("no scene: call atx_renderer_set_scene first");
+        return ATX_ERR_STATE;
+    }
+    ATX_CHECK_ARG(r->settings.max_bounces >= 0, "max_bounces must be >= 0");
+
+    PlainArgs a{};
+    a.sc.sph = r->sph.as<float4>();
+    a.sc.sph_mat = r->sph_mat.as<int32_t>();
+    a.sc.mat = r->mat.as<float4>();
+    a.sc.light = r->light.as<float4>();
+    a.sc.n_sph = r->n_sph;
+    a.sc.n_mat = r->n_mat;
+    a.sc.n_light = r->n_light;
+    camera_inverse_matrices(cam->position, cam->direction, cam->fov_deg, cam->near_clip, cam->far_clip,
+                            cam->width, cam->height, a.cam.inv_proj, a.cam.inv_view);
+    memcpy(a.cam.pos, cam->position, sizeof(a.cam.pos));
+    a.cam.width = r->width;
+    a.cam.height = r->height;
+    a.max_bounces = r->settings.max_bounces;
+    a.sky_light = r->settings.sky_light ? 1u : 0u;
+    a.accum = r->accum.as<float4>();
+    a.comp = r->accumulator == ATX_ACCUM_KAHAN ? r->comp.as<float4>() : nullptr;
+    a.image = r->image.as<uint32_t>();
+    a.counters = r->counters.as<unsigned long long>();
+
+    const size_t npx = (size_t)r->width * r->height;
+    DevBuf d_rad, d_hit, d_sh;
+    uint32_t variant = 0;
+    if ((size_t)r->n_sph * sizeof(float4) <= r->lds_budget) variant |= kVarLds;
+    if (r->accumulator == ATX_ACCUM_KAHAN) variant |= kVarKahan;
+    if (r->exact_loop) variant |= kVarExactLoop;
+    if (dbg_rad || dbg_hit || dbg_shadow) {
+        variant |= kVarDebug;
+        if (dbg_rad) { ATX_HIP(d_rad.reserve(npx * sizeof(float4))); a.dbg_radiance = d_rad.as<float4>(); }
+        if (dbg_hit) { ATX_HIP(d_hit.reserve(npx * dbg_bounces * 4)); a.dbg_hit = d_hit.as<int32_t>(); }
+        if (dbg_shadow) { ATX_HIP(d_sh.reserve(npx * dbg_bounces * 4)); a.dbg_shadow = d_sh.as<int32_t>(); }
+        a.dbg_bounces = dbg_bounces;
+    }
+
+    ATX_HIP(hipMemsetAsync(r->counters.get(), 0, kCtrCount * sizeof(unsigned long long), r->stream));
+    ATX_HIP(hipEventRecord(r->ev0, r->stream));
+    for (uint32_t f = 0; f < frames; ++f) {
+        a.frame = r->frame_index;
+        ATX_HIP(launch_plain(a, variant, r->stream));
+        // Renderer.cu:245-248
+        if (r->settings.accumulation) r->frame_index++;
+        else r->frame_index = 1;
+    }
+    ATX_HIP(hipEventRecord(r->ev1, r->stream));
+    unsigned long long ctr[kCtrCount];
+    ATX_HIP(hipMemcpyAsync(ctr, r->counters.get(), sizeof(ctr), hipMemcpyDeviceToHost, r->stream));
+    if (dbg_rad) ATX_HIP(hipMemcpyAsync(dbg_rad, d_rad.get(), npx * sizeof(float4), hipMemcpyDeviceToHost, r->stream));
+    if (dbg_hit) ATX_HIP(hipMemcpyAsync(dbg_hit, d_hit.get(), npx * dbg_bounces * 4, hipMemcpyDeviceToHost, r->stream));
+    if (dbg_shadow) ATX_HIP(hipMemcpyAsync(dbg_shadow, d_sh.get(), npx * dbg_bounces * 4, hipMemcpyDeviceToHost, r->stream));
+    ATX_HIP(hipStreamSynchronize(r->stream));
+    float ms = 0.0f;
+    ATX_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->stats.samples = (uint64_t)npx * frames;
+    r->stats.rays = ctr[kCtrRays];
+    r->stats.sphere_tests = ctr[kCtrRays] * (uint64_t)r->n_sph;
+    r->stats.exact_retraces = ctr[kCtrExact];
+    r->stats.gpu_ms = ms;
+    return ATX_OK;
+}
+
+}  // namespace atx
+
+using namespace atx;
+
+extern "C" {
+
+int atx_renderer_create(atx_renderer** out)
+{
+    ATX_CHECK_ARG(out, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device available");
+        return ATX_ERR_NODEVICE;
+    }
+    Renderer* r = new (std::nothrow) Renderer();
+    if (!r) { set_error("out of host memory"); return ATX_ERR_NOMEM; }
+    hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev1);
+    if (e == hipSuccess) e = r->counters.reserve(kCtrCount * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        set_error("renderer create: %s", hipGetErrorString(e));
+        atx_renderer_destroy(reinterpret_cast<atx_renderer*>(r));
+        return ATX_ERR_HIP;
+    }
+    if (const char* env = getenv("ATX_EXACT_LOOP")) r->exact_loop = atoi(env) != 0;
+    *out = reinterpret_cast<atx_renderer*>(r);
+    return ATX_OK;
+}
+
+int atx_renderer_destroy(atx_renderer* h)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    if (!r) return ATX_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->ev0) (void)hipEventDestroy(r->ev0);
+    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return ATX_OK;
+}
+
+int atx_renderer_on_resize(atx_renderer* h, uint32_t width, uint32_t height)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(width > 0 && height > 0, "Width or height cannot be zero");
+    if (r->width == width && r->height == height) return ATX_OK;  // Renderer.cu:100-101
+    const size_t npx = (size_t)width * height;
+    ATX_HIP(hipStreamSynchronize(r->stream));
+    r->image.release(); r->accum.release(); r->comp.release();
+    ATX_HIP(r->image.reserve(npx * sizeof(uint32_t)));
+    ATX_HIP(r->accum.reserve(npx * sizeof(float4)));
+    ATX_HIP(r->comp.reserve(npx * sizeof(float4)));
+    r->width = width;
+    r->height = height;
+    r->frame_index = 1;
+    return ATX_OK;
+}
+
+int atx_renderer_set_scene(atx_renderer* h, const atx_sphere* spheres, uint64_t num_spheres,
+                           const atx_material* materials, uint64_t num_materials, const atx_light* lights,
+                           uint64_t num_lights)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    HostSceneLayout L;
+    int rc = build_scene_layout(spheres, num_spheres, materials, num_materials, lights, num_lights, &L);
+    if (rc != ATX_OK) return rc;
+    ATX_HIP(hipStreamSynchronize(r->stream));
+    ATX_HIP(r->sph.reserve(L.sph.size() * 4 + 16));
+    ATX_HIP(r->sph_mat.reserve(L.sph_mat.size() * 4 + 16));
+    ATX_HIP(r->mat.reserve(L.mat.size() * 4 + 16));
+    ATX_HIP(r->light.reserve(L.light.size() * 4 + 16));
+    if (!L.sph.empty()) ATX_HIP(hipMemcpy(r->sph.get(), L.sph.data(), L.sph.size() * 4, hipMemcpyHostToDevice));
+    if (!L.sph_mat.empty()) ATX_HIP(hipMemcpy(r->sph_mat.get(), L.sph_mat.data(), L.sph_mat.size() * 4, hipMemcpyHostToDevice));
+    if (!L.mat.empty()) ATX_HIP(hipMemcpy(r->mat.get(), L.mat.data(), L.mat.size() * 4, hipMemcpyHostToDevice));
+    if (!L.light.empty()) ATX_HIP(hipMemcpy(r->light.get(), L.light.data(), L.light.size() * 4, hipMemcpyHostToDevice));
+    r->n_sph = (uint32_t)num_spheres;
+    r->n_mat = (uint32_t)num_materials;
+    r->n_light = (uint32_t)num_lights;
+    r->has_scene = true;
+    return ATX_OK;
+}
+
+int atx_renderer_set_settings(atx_renderer* h, const atx_settings* s)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r && s, "NULL argument");
+    ATX_CHECK_ARG(s->max_bounces >= 0, "max_bounces must be >= 0");
+    r->settings = *s;
+    return ATX_OK;
+}
+
+int atx_renderer_get_settings(const atx_renderer* h, atx_settings* s)
+{
+    const Renderer* r = reinterpret_cast<const Renderer*>(h);
+    ATX_CHECK_ARG(r && s, "NULL argument");
+    *s = r->settings;
+    return ATX_OK;
+}
+
+int atx_renderer_reset_frame_index(atx_renderer* h)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    r->frame_index = 1;
+    return ATX_OK;
+}
+
+int atx_renderer_get_frame_index(const atx_renderer* h, uint32_t* fi)
+{
+    const Renderer* r = reinterpret_cast<const Renderer*>(h);
+    ATX_CHECK_ARG(r && fi, "NULL argument");
+    *fi = r->frame_index;
+    return ATX_OK;
+}
+
+int atx_renderer_set_accumulator(atx_renderer* h, int kind)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(kind == ATX_ACCUM_NAIVE || kind == ATX_ACCUM_KAHAN, "unknown accumulator kind %d", kind);
+    if (kind != r->accumulator) r->frame_index = 1;  // the buffers' meaning changes
+    r->accumulator = kind;
+    return ATX_OK;
+}
+
+int atx_renderer_render(atx_renderer* h, const atx_camera* cam)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    return renderer_launch_frames(r, cam, 1, nullptr, nullptr, nullptr, 0);
+}
+
+int atx_renderer_render_frames(atx_renderer* h, const atx_camera* cam, uint32_t frames)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    return renderer_launch_frames(r, cam, frames, nullptr, nullptr, nullptr, 0);
+}
+
+int atx_renderer_render_debug(atx_renderer* h, const atx_camera* cam, float* radiance, int32_t* hit_ids,
+                              int32_t* shadow_ids, uint32_t max_bounces)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(!(hit_ids || shadow_ids) || max_bounces > 0, "max_bounces must be > 0 for hit capture");
+    return renderer_launch_frames(r, cam, 1, radiance, hit_ids, shadow_ids, max_bounces);
+}
+
+int atx_renderer_get_image(atx_renderer* h, uint32_t* rgba, uint64_t n)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r && rgba, "NULL argument");
+    const uint64_t npx = (uint64_t)r->width * r->height;
+    ATX_CHECK_ARG(n >= npx, "buffer holds %llu pixels, image has %llu", (unsigned long long)n,
+                  (unsigned long long)npx);
+    if (npx == 0) return ATX_OK;
+    ATX_HIP(hipMemcpyAsync(rgba, r->image.get(), npx * 4, hipMemcpyDeviceToHost, r->stream));
+    ATX_HIP(hipStreamSynchronize(r->stream));
+    return ATX_OK;
+}
+
+int atx_renderer_get_accumulation(atx_renderer* h, float* rgba, uint64_t n)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r && rgba, "NULL argument");
+    const uint64_t nf = 4ull * r->width * r->height;
+    ATX_CHECK_ARG(n >= nf, "buffer holds %llu floats, need %llu", (unsigned long long)n, (unsigned long long)nf);
+    if (nf == 0) return ATX_OK;
+    ATX_HIP(hipMemcpyAsync(rgba, r->accum.get(), nf * 4, hipMemcpyDeviceToHost, r->stream));
+    ATX_HIP(hipStreamSynchronize(r->stream));
+    return ATX_OK;
+}
+
+int atx_renderer_get_stats(const atx_renderer* h, atx_render_stats* s)
+{
+    const Renderer* r = reinterpret_cast<const Renderer*>(h);
+    ATX_CHECK_ARG(r && s, "NULL argument");
+    *s = r->stats;
+    return ATX_OK;
+}
+
+}  // extern "C"

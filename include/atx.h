@@ -1,0 +1,142 @@
+/*
+ * atx.h -- C ABI of the MI355X-native Ataraxia path tracer (libatx.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes.
+ * Every entry point returns ATX_OK (0) or a negative ATX_ERR_* code and never
+ * exits the process; atx_last_error() returns a thread-local message for the
+ * last failing call on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   atx_renderer_create / destroy        Renderer::Renderer / ~Renderer      Engine/include/Renderer.h:20-21
+ *   atx_renderer_on_resize               Renderer::onResize                  Engine/src/Renderer.cu:98-146
+ *   atx_renderer_set_scene               Renderer::allocateDeviceMemory      Engine/src/Renderer.cu:25-57
+ *   atx_renderer_render                  Renderer::Render                    Engine/src/Renderer.cu:173-249
+ *   atx_renderer_get_image               Renderer::getImage + Image pixels   Engine/include/Renderer.h:23, Renderer.cu:240-242
+ *   atx_renderer_get/set_settings        Renderer::getSettings/setSettings   Engine/include/Renderer.h:25-26
+ *   atx_renderer_reset_frame_index       Renderer::resetFrameIndex           Engine/include/Renderer.h:28
+ *   atx_camera_ray_directions            Camera::UpdateRayDirection          Engine/src/Camera.cpp:161-195
+ *   atx_scene_* (scene.json)             Utils::importScene / exportScene    Engine/src/Utils.cpp:53-62,175-187
+ *
+ * Struct layouts equal the reference's packed GLM structs so that arrays can
+ * be handed over without conversion (Scene.h:11-56, SceneNode.h:11-21).
+ */
+#ifndef ATX_H
+#define ATX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATX_OK 0
+#define ATX_ERR_INVALID (-1)   /* bad argument or size mismatch            */
+#define ATX_ERR_HIP (-2)       /* HIP runtime error (message has details) */
+#define ATX_ERR_NOMEM (-3)     /* device or host allocation failed        */
+#define ATX_ERR_STATE (-4)     /* call not valid in the current state     */
+#define ATX_ERR_IO (-5)        /* file could not be read/written/parsed   */
+#define ATX_ERR_BUDGET (-6)    /* request exceeds the HBM budget          */
+#define ATX_ERR_NOTFOUND (-7)  /* unknown id                               */
+#define ATX_ERR_NODEVICE (-8)  /* no HIP device available                  */
+
+/* Sphere {vec3 center; float radius; int id}           SceneNode.h:11-21   (20 B) */
+typedef struct atx_sphere { float center[3]; float radius; int32_t material; } atx_sphere;
+/* Material {albedo, roughness, metallic, F0, emissionColor, emissionIntensity, id}
+ *                                                        Scene.h:28-47      (52 B) */
+typedef struct atx_material {
+    float albedo[3];
+    float roughness;
+    float metallic;
+    float F0[3];
+    float emission_color[3];
+    float emission_intensity;
+    int32_t id;
+} atx_material;
+/* Light {vec3 position; vec3 color; float intensity}    Scene.h:17-26      (28 B) */
+typedef struct atx_light { float position[3]; float color[3]; float intensity; } atx_light;
+/* Settings {bool accumulation; bool skyLight; int maxBounces}  Scene.h:49-56  (8 B) */
+typedef struct atx_settings {
+    uint8_t accumulation;
+    uint8_t sky_light;
+    uint8_t reserved[2];
+    int32_t max_bounces;
+} atx_settings;
+/* Camera state that determines the rays: position, direction, fov (degrees),
+ * near/far clip and the viewport size (Camera.h:60-88).                      */
+typedef struct atx_camera {
+    float position[3];
+    float direction[3];
+    float fov_deg;
+    float near_clip;
+    float far_clip;
+    uint32_t width;
+    uint32_t height;
+} atx_camera;
+
+/* Accumulation buffer arithmetic.  NAIVE is the reference's `accum += c`
+ * (Renderer.cu:165); KAHAN keeps a compensation term per channel.           */
+#define ATX_ACCUM_NAIVE 0
+#define ATX_ACCUM_KAHAN 1
+
+/* Counters of the last render call. */
+typedef struct atx_render_stats {
+    uint64_t samples;        /* path samples (pixels x frames)                 */
+    uint64_t rays;           /* rays traced: primary/continuation + shadow      */
+    uint64_t sphere_tests;   /* rays x spheres: ray-sphere tests performed      */
+    uint64_t exact_retraces; /* traces resolved by the exact fallback loop      */
+    double gpu_ms;           /* HIP-event time of the render kernels            */
+} atx_render_stats;
+
+const char* atx_last_error(void);
+const char* atx_version(void);
+int atx_device_count(int* count);
+int atx_device_synchronize(void);
+
+/* ---- Renderer: single scene, one frame per Render (the plain kernel) ----- */
+typedef struct atx_renderer atx_renderer;
+
+int atx_renderer_create(atx_renderer** out);
+int atx_renderer_destroy(atx_renderer* r);
+/* Allocates the RGBA8 image and accumulation for width x height and resets
+ * the frame index to 1 when the size changes (Renderer.cu:98-146).          */
+int atx_renderer_on_resize(atx_renderer* r, uint32_t width, uint32_t height);
+/* Uploads a flattened scene (copied).  Material ids outside [0, m) are
+ * clamped to 0, as Renderer.cu:30-37 does.  Does not reset the frame index. */
+int atx_renderer_set_scene(atx_renderer* r, const atx_sphere* spheres, uint64_t num_spheres,
+                           const atx_material* materials, uint64_t num_materials,
+                           const atx_light* lights, uint64_t num_lights);
+int atx_renderer_set_settings(atx_renderer* r, const atx_settings* s);
+int atx_renderer_get_settings(const atx_renderer* r, atx_settings* s);
+int atx_renderer_reset_frame_index(atx_renderer* r);
+int atx_renderer_get_frame_index(const atx_renderer* r, uint32_t* frame_index);
+int atx_renderer_set_accumulator(atx_renderer* r, int kind);
+/* One frame: clears the accumulation when the frame index is 1, traces one
+ * path per pixel, accumulates, writes RGBA8, then advances the frame index
+ * (or keeps it at 1 when accumulation is off).  Returns after the GPU is done. */
+int atx_renderer_render(atx_renderer* r, const atx_camera* cam);
+/* `frames` consecutive Render calls queued back to back on the renderer's
+ * stream with a single synchronisation at the end (no readback).           */
+int atx_renderer_render_frames(atx_renderer* r, const atx_camera* cam, uint32_t frames);
+/* Copies the RGBA8 image ((a<<24)|(b<<16)|(g<<8)|r per pixel) to the host. */
+int atx_renderer_get_image(atx_renderer* r, uint32_t* rgba, uint64_t num_pixels);
+/* Copies the accumulation buffer (4 floats per pixel: r, g, b, frames). */
+int atx_renderer_get_accumulation(atx_renderer* r, float* rgba, uint64_t num_floats);
+int atx_renderer_get_stats(const atx_renderer* r, atx_render_stats* stats);
+/* One frame with per-pixel debug capture (parity tests): radiance of this
+ * frame's sample (4 floats per pixel), closest-hit index of the continuation
+ * ray and of the shadow ray of every bounce (-1 miss, -2 not traced), each
+ * array width*height*max_bounces, pixel-major.  Any pointer may be NULL.     */
+int atx_renderer_render_debug(atx_renderer* r, const atx_camera* cam, float* radiance,
+                              int32_t* hit_ids, int32_t* shadow_ids, uint32_t max_bounces);
+
+/* ---- Camera: host-side ray directions (Camera::UpdateRayDirection) -------- */
+/* Writes width*height*3 floats, bit-identical to what the kernels generate. */
+int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_floats);
+/* Inverse projection and inverse view matrices (column-major, 16 floats each). */
+int atx_camera_matrices(const atx_camera* cam, float* inv_proj, float* inv_view);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
