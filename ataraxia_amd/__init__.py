@@ -4,13 +4,15 @@ The compute path is libatx.so (hand-written HIP kernels behind the C ABI in
 include/atx.h).  This package is the Python view of that ABI used by the
 tests and the benchmark; it has no CPU rendering path of its own.
 """
-from ._lib import (ATX_ACCUM_KAHAN, ATX_ACCUM_NAIVE, AtxError, LIB_PATH, device_count, last_error)
+from ._lib import (ATX_ACCUM_KAHAN, ATX_ACCUM_NAIVE, ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, AtxError, LIB_PATH,
+                   device_count, last_error)
+from .batch import Batch
 from .renderer import Renderer, camera_matrices, camera_ray_directions
 from .scene import (Camera, FlatScene, Settings, SplitMix64, batch_mix_scenes, default_scene,
                     plain_1080p_scene, plain_cpu_scene, precision_scene, serve_request_scene)
 
 __all__ = [
-    "ATX_ACCUM_KAHAN", "ATX_ACCUM_NAIVE", "AtxError", "LIB_PATH", "device_count", "last_error",
+    "ATX_ACCUM_KAHAN", "ATX_ACCUM_NAIVE", "ATX_KERNEL_PLAIN", "ATX_KERNEL_WAVEFRONT", "AtxError", "Batch", "LIB_PATH", "device_count", "last_error",
     "Renderer", "camera_matrices", "camera_ray_directions", "Camera", "FlatScene", "Settings",
     "SplitMix64", "batch_mix_scenes", "default_scene", "plain_1080p_scene", "plain_cpu_scene",
     "precision_scene", "serve_request_scene",

@@ -33,6 +33,8 @@ ATX_ERR_NODEVICE = -8
 
 ATX_ACCUM_NAIVE = 0
 ATX_ACCUM_KAHAN = 1
+ATX_KERNEL_PLAIN = 0
+ATX_KERNEL_WAVEFRONT = 1
 
 
 class Sphere(C.Structure):
@@ -58,6 +60,17 @@ class CameraDesc(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("fov_deg", C.c_float),
                 ("near_clip", C.c_float), ("far_clip", C.c_float), ("width", C.c_uint32),
                 ("height", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("spheres", C.c_void_p), ("num_spheres", C.c_uint64), ("materials", C.c_void_p),
+                ("num_materials", C.c_uint64), ("lights", C.c_void_p), ("num_lights", C.c_uint64),
+                ("settings", Settings), ("camera", CameraDesc), ("spp", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("chunk_px", C.c_uint32), ("workgroups", C.c_uint32), ("use_graph", C.c_uint32),
+                ("frames_per_launch", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -102,6 +115,21 @@ PROTOTYPES = {
     "atx_renderer_get_accumulation": (_I, [_P, _P, _U64]),
     "atx_renderer_get_stats": (_I, [_P, C.POINTER(RenderStats)]),
     "atx_renderer_render_debug": (_I, [_P, C.POINTER(CameraDesc), _P, _P, _P, _U32]),
+    "atx_renderer_set_kernel": (_I, [_P, _I]),
+    "atx_batch_create": (_I, [C.POINTER(_P)]),
+    "atx_batch_destroy": (_I, [_P]),
+    "atx_batch_add_scene": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(_U32)]),
+    "atx_batch_set_accumulator": (_I, [_P, _I]),
+    "atx_batch_set_tuning": (_I, [_P, C.POINTER(BatchTuning)]),
+    "atx_batch_get_tuning": (_I, [_P, C.POINTER(BatchTuning)]),
+    "atx_batch_autotune": (_I, [_P, C.c_char_p, C.POINTER(BatchTuning)]),
+    "atx_batch_render": (_I, [_P]),
+    "atx_batch_render_frames": (_I, [_P, _U32]),
+    "atx_batch_capture": (_I, [_P, _U32, _U32]),
+    "atx_batch_get_capture": (_I, [_P, _U32, _P, _P, _P]),
+    "atx_batch_get_image": (_I, [_P, _U32, _P, _U64]),
+    "atx_batch_get_accumulation": (_I, [_P, _U32, _P, _U64]),
+    "atx_batch_get_stats": (_I, [_P, C.POINTER(RenderStats)]),
     "atx_camera_ray_directions": (_I, [C.POINTER(CameraDesc), _P, _U64]),
     "atx_camera_matrices": (_I, [C.POINTER(CameraDesc), _P, _P]),
 }

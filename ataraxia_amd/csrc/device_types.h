@@ -40,6 +40,7 @@ struct CameraDev {
 enum CounterSlot : uint32_t {
     kCtrRays = 0,          // rays traced (continuation/primary + shadow)
     kCtrExact = 1,         // traces resolved by the exact fallback loop
+    kCtrTests = 2,         // ray-sphere tests (rays x spheres of their scene)
     kCtrCount = 4,
 };
 

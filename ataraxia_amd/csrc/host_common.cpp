@@ -64,6 +64,33 @@ int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material
     return ATX_OK;
 }
 
+size_t scene_device_bytes(uint64_t n, uint64_t m, uint64_t l)
+{
+    return (16 * n + 16) + (4 * n + 16) + (48 * m + 16) + (32 * l + 16);
+}
+
+int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* materials, uint64_t m,
+                 const atx_light* lights, uint64_t l, DevScene* out)
+{
+    HostSceneLayout L;
+    int rc = build_scene_layout(spheres, n, materials, m, lights, l, &L);
+    if (rc != ATX_OK) return rc;
+    ATX_HIP(out->sph.reserve(L.sph.size() * 4 + 16));
+    ATX_HIP(out->sph_mat.reserve(L.sph_mat.size() * 4 + 16));
+    ATX_HIP(out->mat.reserve(L.mat.size() * 4 + 16));
+    ATX_HIP(out->light.reserve(L.light.size() * 4 + 16));
+    if (!L.sph.empty()) ATX_HIP(hipMemcpy(out->sph.get(), L.sph.data(), L.sph.size() * 4, hipMemcpyHostToDevice));
+    if (!L.sph_mat.empty())
+        ATX_HIP(hipMemcpy(out->sph_mat.get(), L.sph_mat.data(), L.sph_mat.size() * 4, hipMemcpyHostToDevice));
+    if (!L.mat.empty()) ATX_HIP(hipMemcpy(out->mat.get(), L.mat.data(), L.mat.size() * 4, hipMemcpyHostToDevice));
+    if (!L.light.empty())
+        ATX_HIP(hipMemcpy(out->light.get(), L.light.data(), L.light.size() * 4, hipMemcpyHostToDevice));
+    out->n_sph = (uint32_t)n;
+    out->n_mat = (uint32_t)m;
+    out->n_light = (uint32_t)l;
+    return ATX_OK;
+}
+
 }  // namespace atx
 
 using namespace atx;

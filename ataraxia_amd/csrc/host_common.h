@@ -93,4 +93,18 @@ struct HostSceneLayout {
 int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material* materials,
                        uint64_t m, const atx_light* lights, uint64_t l, HostSceneLayout* out);
 
+// One scene's geometry in HBM (device layout of device_types.h).
+struct DevScene {
+    DevBuf sph, sph_mat, mat, light;
+    uint32_t n_sph = 0, n_mat = 0, n_light = 0;
+    size_t bytes() const { return sph.bytes() + sph_mat.bytes() + mat.bytes() + light.bytes(); }
+};
+
+// Validates, converts and uploads (synchronously) a flattened scene.
+int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* materials, uint64_t m,
+                 const atx_light* lights, uint64_t l, DevScene* out);
+
+// Device bytes a scene of this size occupies (geometry only).
+size_t scene_device_bytes(uint64_t n, uint64_t m, uint64_t l);
+
 }  // namespace atx

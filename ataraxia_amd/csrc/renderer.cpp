@@ -13,18 +13,24 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 #include "camera_host.h"
 #include "device_types.h"
 #include "host_common.h"
+#include "wavefront_host.h"
 
 namespace atx {
 
 struct Renderer {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf sph, sph_mat, mat, light;
-    uint32_t n_sph = 0, n_mat = 0, n_light = 0;
+    DevScene geo;
     bool has_scene = false;
+    int kernel = ATX_KERNEL_PLAIN;
+    DevBuf wf_table, wf_queue;
+    uint32_t wf_chunk = kDefaultChunkPx;
+    uint32_t wf_frames = kDefaultFramesPerLaunch;
     DevBuf accum, comp, image, counters;
     uint32_t width = 0, height = 0;
     atx_settings settings{1, 0, {0, 0}, 15};  // Scene.h:49-56 defaults
@@ -53,13 +59,13 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     ATX_CHECK_ARG(r->settings.max_bounces >= 0, "max_bounces must be >= 0");
 
     PlainArgs a{};
-    a.sc.sph = r->sph.as<float4>();
-    a.sc.sph_mat = r->sph_mat.as<int32_t>();
-    a.sc.mat = r->mat.as<float4>();
-    a.sc.light = r->light.as<float4>();
-    a.sc.n_sph = r->n_sph;
-    a.sc.n_mat = r->n_mat;
-    a.sc.n_light = r->n_light;
+    a.sc.sph = r->geo.sph.as<float4>();
+    a.sc.sph_mat = r->geo.sph_mat.as<int32_t>();
+    a.sc.mat = r->geo.mat.as<float4>();
+    a.sc.light = r->geo.light.as<float4>();
+    a.sc.n_sph = r->geo.n_sph;
+    a.sc.n_mat = r->geo.n_mat;
+    a.sc.n_light = r->geo.n_light;
     camera_inverse_matrices(cam->position, cam->direction, cam->fov_deg, cam->near_clip, cam->far_clip,
                             cam->width, cam->height, a.cam.inv_proj, a.cam.inv_view);
     memcpy(a.cam.pos, cam->position, sizeof(a.cam.pos));
@@ -75,7 +81,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     const size_t npx = (size_t)r->width * r->height;
     DevBuf d_rad, d_hit, d_sh;
     uint32_t variant = 0;
-    if ((size_t)r->n_sph * sizeof(float4) <= r->lds_budget) variant |= kVarLds;
+    if ((size_t)r->geo.n_sph * sizeof(float4) <= r->lds_budget) variant |= kVarLds;
     if (r->accumulator == ATX_ACCUM_KAHAN) variant |= kVarKahan;
     if (r->exact_loop) variant |= kVarExactLoop;
     if (dbg_rad || dbg_hit || dbg_shadow) {
@@ -86,14 +92,64 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         a.dbg_bounces = dbg_bounces;
     }
 
+    // wavefront mode: launches of up to wf_frames frames each (one frame per
+    // launch when accumulation is off: the frame index then stays 1)
+    std::vector<WfScene> table;
+    std::vector<uint32_t> launch_frames;
+    const uint32_t chunk = r->wf_chunk;
+    const bool wf = r->kernel == ATX_KERNEL_WAVEFRONT && frames > 0;
+    if (wf) {
+        uint32_t fi = r->frame_index, left = frames;
+        while (left > 0) {
+            const uint32_t k = r->settings.accumulation ? (left < r->wf_frames ? left : r->wf_frames) : 1u;
+            WfScene e;
+            fill_wf_scene(r->geo, r->settings, *cam, a.accum, a.comp, a.image, a.comp != nullptr, fi, k, &e);
+            e.chunk_begin = 0;
+            e.n_chunks = chunks_for((uint32_t)npx, chunk);
+            e.dbg_radiance = a.dbg_radiance;
+            e.dbg_hit = a.dbg_hit;
+            e.dbg_shadow = a.dbg_shadow;
+            e.dbg_bounces = a.dbg_bounces;
+            table.push_back(e);
+            launch_frames.push_back(k);
+            fi = r->settings.accumulation ? fi + k : 1u;
+            left -= k;
+        }
+        ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
+        ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
+        ATX_HIP(hipMemcpyAsync(r->wf_table.get(), table.data(), table.size() * sizeof(WfScene),
+                               hipMemcpyHostToDevice, r->stream));
+    }
+
     ATX_HIP(hipMemsetAsync(r->counters.get(), 0, kCtrCount * sizeof(unsigned long long), r->stream));
+    if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
-    for (uint32_t f = 0; f < frames; ++f) {
-        a.frame = r->frame_index;
-        ATX_HIP(launch_plain(a, variant, r->stream));
-        // Renderer.cu:245-248
-        if (r->settings.accumulation) r->frame_index++;
-        else r->frame_index = 1;
+    if (wf) {
+        const uint32_t grid = wavefront_full_grid(chunk, a.comp != nullptr);
+        for (size_t l = 0; l < table.size(); ++l) {
+            WfPass wp{};
+            wp.scenes = r->wf_table.as<WfScene>() + l;
+            wp.n_scenes = 1;
+            wp.total_chunks = table[l].n_chunks;
+            wp.chunk_px = chunk;
+            wp.kahan = a.comp != nullptr ? 1u : 0u;
+            wp.queue = r->wf_queue.as<unsigned int>() + l;
+            wp.counters = a.counters;
+            ATX_HIP(launch_wavefront(wp, WfLaunch{grid}, r->stream));
+            // Renderer.cu:245-248, once per frame of the launch
+            for (uint32_t f = 0; f < launch_frames[l]; ++f) {
+                if (r->settings.accumulation) r->frame_index++;
+                else r->frame_index = 1;
+            }
+        }
+    } else {
+        for (uint32_t f = 0; f < frames; ++f) {
+            a.frame = r->frame_index;
+            ATX_HIP(launch_plain(a, variant, r->stream));
+            // Renderer.cu:245-248
+            if (r->settings.accumulation) r->frame_index++;
+            else r->frame_index = 1;
+        }
     }
     ATX_HIP(hipEventRecord(r->ev1, r->stream));
     unsigned long long ctr[kCtrCount];
@@ -106,7 +162,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     ATX_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
     r->stats.samples = (uint64_t)npx * frames;
     r->stats.rays = ctr[kCtrRays];
-    r->stats.sphere_tests = ctr[kCtrRays] * (uint64_t)r->n_sph;
+    r->stats.sphere_tests = wf ? ctr[kCtrTests] : ctr[kCtrRays] * (uint64_t)r->geo.n_sph;
     r->stats.exact_retraces = ctr[kCtrExact];
     r->stats.gpu_ms = ms;
     return ATX_OK;
@@ -139,6 +195,15 @@ int atx_renderer_create(atx_renderer** out)
         return ATX_ERR_HIP;
     }
     if (const char* env = getenv("ATX_EXACT_LOOP")) r->exact_loop = atoi(env) != 0;
+    if (const char* env = getenv("ATX_LDS_BUDGET")) r->lds_budget = (size_t)strtoull(env, nullptr, 10);
+    if (const char* env = getenv("ATX_WF_CHUNK")) {
+        const uint32_t c = (uint32_t)strtoul(env, nullptr, 10);
+        if (c >= 64 && c % 64 == 0 && c <= kMaxChunkPx) r->wf_chunk = c;
+    }
+    if (const char* env = getenv("ATX_WF_FRAMES")) {
+        const uint32_t f = (uint32_t)strtoul(env, nullptr, 10);
+        if (f >= 1 && f <= 4096) r->wf_frames = f;
+    }
     *out = reinterpret_cast<atx_renderer*>(r);
     return ATX_OK;
 }
@@ -179,22 +244,19 @@ int atx_renderer_set_scene(atx_renderer* h, const atx_sphere* spheres, uint64_t 
 {
     Renderer* r = reinterpret_cast<Renderer*>(h);
     ATX_CHECK_ARG(r, "renderer is NULL");
-    HostSceneLayout L;
-    int rc = build_scene_layout(spheres, num_spheres, materials, num_materials, lights, num_lights, &L);
-    if (rc != ATX_OK) return rc;
     ATX_HIP(hipStreamSynchronize(r->stream));
-    ATX_HIP(r->sph.reserve(L.sph.size() * 4 + 16));
-    ATX_HIP(r->sph_mat.reserve(L.sph_mat.size() * 4 + 16));
-    ATX_HIP(r->mat.reserve(L.mat.size() * 4 + 16));
-    ATX_HIP(r->light.reserve(L.light.size() * 4 + 16));
-    if (!L.sph.empty()) ATX_HIP(hipMemcpy(r->sph.get(), L.sph.data(), L.sph.size() * 4, hipMemcpyHostToDevice));
-    if (!L.sph_mat.empty()) ATX_HIP(hipMemcpy(r->sph_mat.get(), L.sph_mat.data(), L.sph_mat.size() * 4, hipMemcpyHostToDevice));
-    if (!L.mat.empty()) ATX_HIP(hipMemcpy(r->mat.get(), L.mat.data(), L.mat.size() * 4, hipMemcpyHostToDevice));
-    if (!L.light.empty()) ATX_HIP(hipMemcpy(r->light.get(), L.light.data(), L.light.size() * 4, hipMemcpyHostToDevice));
-    r->n_sph = (uint32_t)num_spheres;
-    r->n_mat = (uint32_t)num_materials;
-    r->n_light = (uint32_t)num_lights;
+    int rc = upload_scene(spheres, num_spheres, materials, num_materials, lights, num_lights, &r->geo);
+    if (rc != ATX_OK) return rc;
     r->has_scene = true;
+    return ATX_OK;
+}
+
+int atx_renderer_set_kernel(atx_renderer* h, int kind)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(kind == ATX_KERNEL_PLAIN || kind == ATX_KERNEL_WAVEFRONT, "unknown kernel kind %d", kind);
+    r->kernel = kind;
     return ATX_OK;
 }
 

@@ -1,0 +1,66 @@
+// wavefront_host.h -- host helpers that fill the persistent kernel's scene
+// table entries (shared by the batch renderer, the server and the Renderer's
+// wavefront mode).
+#pragma once
+
+#include <string.h>
+
+#include "camera_host.h"
+#include "host_common.h"
+#include "wavefront.h"
+
+namespace atx {
+
+inline void fill_wf_scene(const DevScene& g, const atx_settings& st, const atx_camera& cam, float4* accum,
+                          float4* comp, uint32_t* image, bool kahan, uint32_t frame0, uint32_t frames,
+                          WfScene* out)
+{
+    memset(out, 0, sizeof(*out));
+    out->sph = g.sph.as<float4>();
+    out->sph_mat = g.sph_mat.as<int32_t>();
+    out->mat = g.mat.as<float4>();
+    out->light = g.light.as<float4>();
+    out->n_sph = g.n_sph;
+    out->n_mat = g.n_mat;
+    out->n_light = g.n_light;
+    out->width = cam.width;
+    out->height = cam.height;
+    out->max_bounces = st.max_bounces;
+    out->sky_light = st.sky_light ? 1u : 0u;
+    out->frame0 = frame0;
+    out->frames = frames;
+    out->kahan = kahan ? 1u : 0u;
+    camera_inverse_matrices(cam.position, cam.direction, cam.fov_deg, cam.near_clip, cam.far_clip, cam.width,
+                            cam.height, out->inv_proj, out->inv_view);
+    out->cam_pos[0] = cam.position[0];
+    out->cam_pos[1] = cam.position[1];
+    out->cam_pos[2] = cam.position[2];
+    out->accum = accum;
+    out->comp = comp;
+    out->image = image;
+}
+
+inline uint32_t chunks_for(uint32_t pixels, uint32_t chunk_items)
+{
+    return (pixels + chunk_items - 1) / chunk_items;
+}
+
+inline uint32_t device_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return (uint32_t)cus;
+}
+
+// Persistent grid that fills the device: CUs x resident workgroups per CU.
+inline uint32_t wavefront_full_grid(uint32_t chunk_px, bool kahan)
+{
+    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu(chunk_px, kahan);
+}
+
+constexpr uint32_t kDefaultChunkPx = 128;
+constexpr uint32_t kDefaultFramesPerLaunch = 16;
+
+}  // namespace atx

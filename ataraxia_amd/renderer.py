@@ -82,6 +82,10 @@ class Renderer:
     def set_accumulator(self, kind: int) -> None:
         check(lib.atx_renderer_set_accumulator(self._h, kind))
 
+    def set_kernel(self, kind: int) -> None:
+        """ATX_KERNEL_PLAIN (one lane per pixel) or ATX_KERNEL_WAVEFRONT."""
+        check(lib.atx_renderer_set_kernel(self._h, kind))
+
     # -- Renderer::Render ----------------------------------------------------------
     def render(self, camera: Camera, scene: FlatScene) -> None:
         if self._scene_ref is not scene or self.frame_index == 1:

@@ -30,6 +30,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time budget of the CPU baseline")
     ap.add_argument("--exact-loop", action="store_true", help="use the exact (slow) closest-hit loop")
+    ap.add_argument("--kernel", choices=["plain", "wavefront"], default="plain")
     return ap.parse_args()
 
 
@@ -67,11 +68,12 @@ def main():
         sys.exit(2)
     if args.exact_loop:
         os.environ["ATX_EXACT_LOOP"] = "1"
-    from ataraxia_amd import Renderer, plain_1080p_scene
+    from ataraxia_amd import ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Renderer, plain_1080p_scene
 
     sc = plain_1080p_scene()
     cam = sc.camera
     r = Renderer()
+    r.set_kernel(ATX_KERNEL_WAVEFRONT if args.kernel == "wavefront" else ATX_KERNEL_PLAIN)
     r.on_resize(cam.width, cam.height)
     r.set_settings(sc.settings)
     r.set_scene(sc)
@@ -104,7 +106,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded splitmix64 scene generator)",
         "config": {"workload": "plain_1080p: 1920x1080, 256 spheres + ground, 8 lights, 8 bounces, "
-                               "1 spp/frame accumulated", "kernel": "plain single-scene",
+                               "1 spp/frame accumulated", "kernel": args.kernel,
                    "spheres": int(sc.num_spheres), "lights": int(len(sc.lights)), "frames": args.steps},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": None,

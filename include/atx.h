@@ -130,6 +130,62 @@ int atx_renderer_get_stats(const atx_renderer* r, atx_render_stats* stats);
 int atx_renderer_render_debug(atx_renderer* r, const atx_camera* cam, float* radiance,
                               int32_t* hit_ids, int32_t* shadow_ids, uint32_t max_bounces);
 
+/* Kernel used by atx_renderer_render*: PLAIN = one lane per pixel, the
+ * reference's per-pixel loop; WAVEFRONT = the persistent path-regenerating
+ * kernel (results are identical, only the scheduling differs).             */
+#define ATX_KERNEL_PLAIN 0
+#define ATX_KERNEL_WAVEFRONT 1
+int atx_renderer_set_kernel(atx_renderer* r, int kind);
+
+/* ---- Batch: many scenes, persistent launches of several frames each ------ */
+/* No reference counterpart: the reference renders one scene per Render call.
+ * Scene arrays are copied at add time; every scene keeps its own accumulation
+ * and image on the device.                                                  */
+typedef struct atx_scene_desc {
+    const atx_sphere* spheres;
+    uint64_t num_spheres;
+    const atx_material* materials;
+    uint64_t num_materials;
+    const atx_light* lights;
+    uint64_t num_lights;
+    atx_settings settings;
+    atx_camera camera;
+    uint32_t spp;        /* frames (1 sample per pixel each) to accumulate */
+    uint32_t reserved;
+} atx_scene_desc;
+
+typedef struct atx_batch_tuning {
+    uint32_t chunk_px;          /* pixels per work chunk: 64, 128, 192 or 256 (0 = 128)   */
+    uint32_t workgroups;        /* persistent grid, 256-lane workgroups (0 = fill device) */
+    uint32_t use_graph;         /* capture all launches in a HIP graph and replay it       */
+    uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 16)           */
+} atx_batch_tuning;
+
+typedef struct atx_batch atx_batch;
+
+int atx_batch_create(atx_batch** out);
+int atx_batch_destroy(atx_batch* b);
+int atx_batch_add_scene(atx_batch* b, const atx_scene_desc* desc, uint32_t* scene_id);
+int atx_batch_set_accumulator(atx_batch* b, int kind);
+int atx_batch_set_tuning(atx_batch* b, const atx_batch_tuning* t);
+int atx_batch_get_tuning(const atx_batch* b, atx_batch_tuning* t);
+/* Times candidate tunings on this batch (its first frames only) and applies
+ * the fastest; with cache_path != NULL the choice is read from / written to
+ * that file, keyed by device and workload signature.                       */
+int atx_batch_autotune(atx_batch* b, const char* cache_path, atx_batch_tuning* chosen);
+/* Renders every scene from frame 1 to its spp (accumulation restarts).      */
+int atx_batch_render(atx_batch* b);
+/* The same, limited to each scene's first max_frames frames.               */
+int atx_batch_render_frames(atx_batch* b, uint32_t max_frames);
+/* Per-pixel capture of the last rendered frame of one scene (parity tests):
+ * enable before rendering, read after.  Layout as atx_renderer_render_debug. */
+int atx_batch_capture(atx_batch* b, uint32_t scene_id, uint32_t max_bounces);
+int atx_batch_get_capture(atx_batch* b, uint32_t scene_id, float* radiance, int32_t* hit_ids,
+                          int32_t* shadow_ids);
+int atx_batch_get_image(atx_batch* b, uint32_t scene_id, uint32_t* rgba, uint64_t num_pixels);
+int atx_batch_get_accumulation(atx_batch* b, uint32_t scene_id, float* rgba, uint64_t num_floats);
+int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
+
 /* ---- Camera: host-side ray directions (Camera::UpdateRayDirection) -------- */
 /* Writes width*height*3 floats, bit-identical to what the kernels generate. */
 int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_floats);

@@ -1,0 +1,162 @@
+"""GPU parity of the persistent wavefront kernel in its batched form.
+
+* every scene of a mixed batch (different sizes, sphere counts, lights,
+  bounce limits and spp) equals the oracle's frame-by-frame accumulation;
+* the last frame's closest-hit indices equal the oracle's exactly;
+* results do not depend on the tuning (chunk size, grid size, graph or not),
+  on graph replays, or on which other scenes share the batch;
+* on the full 1080p workload the wavefront kernel equals the plain kernel.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ataraxia_amd import (ATX_ACCUM_KAHAN, ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Batch, Camera, Renderer,
+                          Settings, default_scene, plain_1080p_scene, plain_cpu_scene)
+from ataraxia_amd.scene import (FlatScene, SplitMix64, make_lights, make_materials, make_spheres, random_scene,
+                                scene_from_camera_overrides)
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_scenes():
+    out = []
+    s = scene_from_camera_overrides(default_scene(), 37, 23)
+    s.settings = Settings(True, False, 6)
+    s.spp = 3
+    out.append(s)
+    s = scene_from_camera_overrides(plain_cpu_scene(), 64, 36)
+    s.spp = 4
+    out.append(s)
+    s = scene_from_camera_overrides(plain_1080p_scene(), 96, 54)
+    s.spp = 2
+    out.append(s)
+    rng = SplitMix64(31)
+    sph, mat, lig = random_scene(rng, 40, 0, 8.0, (0.0, 3.0), (8.0, 12.0))
+    out.append(FlatScene(sph, mat, lig, Settings(True, True, 3), Camera.looking_at((0, 4, 16), width=30, height=20),
+                         spp=5, name="nolight-sky"))
+    out.append(FlatScene(make_spheres(0), make_materials(0), make_lights(0), Settings(True, True, 2),
+                         Camera(width=1, height=1), spp=1, name="empty-1x1"))
+    sph, mat, lig = random_scene(rng, 12, 3, 6.0, (0.0, 3.0), (8.0, 12.0))
+    out.append(FlatScene(sph, mat, lig, Settings(True, False, 0), Camera.looking_at((0, 3, 12), width=20, height=10),
+                         spp=2, name="zero-bounce"))
+    return out
+
+
+def oracle_accumulate(sc, frames, kahan=False):
+    cam = O.camera_from(sc.camera)
+    osc = O.Scene(sc.spheres, sc.materials, sc.lights)
+    st = O.settings(sc.settings.max_bounces, sc.settings.sky_light, sc.settings.accumulation)
+    n = sc.camera.width * sc.camera.height
+    acc = np.zeros((n, 4), np.float32)
+    comp = np.zeros((n, 4), np.float32) if kahan else None
+    img = None
+    for f in range(1, frames + 1):
+        acc, comp, img, _, _ = O.render_frame(cam, osc, st, f, accum=acc, comp=comp, want_radiance=False)
+    return acc, img
+
+
+def render_batch(scenes, tuning=None, kahan=False, capture=()):
+    with Batch() as b:
+        ids = [b.add_scene(s) for s in scenes]
+        if kahan:
+            b.set_accumulator(ATX_ACCUM_KAHAN)
+        if tuning:
+            b.set_tuning(**tuning)
+        for i in capture:
+            b.capture(ids[i], max(1, scenes[i].settings.max_bounces))
+        st = b.render()
+        accs = [b.get_accumulation(i).reshape(-1, 4).copy() for i in ids]
+        imgs = [b.get_image(i).reshape(-1).copy() for i in ids]
+        caps = {i: b.get_capture(ids[i], max(1, scenes[i].settings.max_bounces)) for i in capture}
+    return accs, imgs, caps, st
+
+
+def test_batch_matches_oracle_per_scene():
+    scenes = mixed_scenes()
+    print("case batch mixed", [s.name for s in scenes], flush=True)
+    accs, imgs, caps, st = render_batch(scenes, capture=(0, 2))
+    assert st["samples"] == sum(s.camera.width * s.camera.height * s.spp for s in scenes)
+    for i, s in enumerate(scenes):
+        oacc, oimg = oracle_accumulate(s, s.spp)
+        assert np.array_equal(accs[i], oacc), s.name
+        assert np.array_equal(imgs[i], oimg), s.name
+    for i in (0, 2):
+        s = scenes[i]
+        nb = max(1, s.settings.max_bounces)
+        rad, hit, sh = caps[i]
+        orad, ohit, osh, _ = O.render_frame_debug(O.camera_from(s.camera), O.Scene(s.spheres, s.materials, s.lights),
+                                                  O.settings(s.settings.max_bounces, s.settings.sky_light), s.spp, nb)
+        assert np.array_equal(hit, ohit) and np.array_equal(sh, osh)
+        assert np.array_equal(rad, orad)
+
+
+def test_batch_kahan_matches_oracle():
+    scenes = mixed_scenes()[:3]
+    accs, _, _, _ = render_batch(scenes, kahan=True)
+    for i, s in enumerate(scenes):
+        oacc, _ = oracle_accumulate(s, s.spp, kahan=True)
+        assert np.array_equal(accs[i], oacc), s.name
+
+
+@pytest.mark.parametrize("tuning", [
+    {"chunk_px": 64, "workgroups": 2, "use_graph": True, "frames_per_launch": 1},
+    {"chunk_px": 64, "workgroups": 1, "use_graph": False, "frames_per_launch": 2},
+    {"chunk_px": 256, "workgroups": 0, "use_graph": True, "frames_per_launch": 64},
+    {"chunk_px": 192, "workgroups": 37, "use_graph": False, "frames_per_launch": 3},
+], ids=["c64-g2-f1", "c64-g1-f2-eager", "c256-full-f64", "c192-g37-f3-eager"])
+def test_batch_independent_of_tuning(tuning):
+    scenes = mixed_scenes()
+    print("case tuning", tuning, flush=True)
+    ref, _, _, _ = render_batch(scenes)
+    got, _, _, _ = render_batch(scenes, tuning=tuning)
+    for a, b_, s in zip(ref, got, scenes):
+        assert np.array_equal(a, b_), s.name
+
+
+def test_graph_replays_and_alone_vs_mix():
+    scenes = mixed_scenes()
+    with Batch() as b:
+        ids = [b.add_scene(s) for s in scenes]
+        b.render()
+        first = [b.get_accumulation(i).copy() for i in ids]
+        for _ in range(3):  # replays restart accumulation at frame 1 -> same bytes
+            b.render()
+            for i in ids:
+                assert np.array_equal(b.get_accumulation(i), first[i])
+    # the same scene alone in a batch
+    for k in (1, 2):
+        alone, _, _, _ = render_batch([scenes[k]])
+        assert np.array_equal(alone[0], first[k].reshape(-1, 4))
+
+
+def test_wavefront_equals_plain_on_full_1080p():
+    sc = plain_1080p_scene()
+    cam = sc.camera
+    out = {}
+    for kernel in (ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT):
+        with Renderer() as r:
+            r.set_kernel(kernel)
+            r.on_resize(cam.width, cam.height)
+            r.set_settings(sc.settings)
+            r.set_scene(sc)
+            r.render_frames(cam, 3)
+            out[kernel] = (r.get_accumulation().copy(), r.get_image().copy(), r.stats()["rays"])
+    assert np.array_equal(out[ATX_KERNEL_PLAIN][0], out[ATX_KERNEL_WAVEFRONT][0])
+    assert np.array_equal(out[ATX_KERNEL_PLAIN][1], out[ATX_KERNEL_WAVEFRONT][1])
+    assert out[ATX_KERNEL_PLAIN][2] == out[ATX_KERNEL_WAVEFRONT][2]
+
+
+def test_batch_argument_errors():
+    from ataraxia_amd import AtxError
+    with Batch() as b:
+        with pytest.raises(AtxError):
+            b.render()  # no scenes
+        bad = scene_from_camera_overrides(plain_cpu_scene(), 8, 8)
+        with pytest.raises(AtxError):
+            b.add_scene(bad, spp=0)
+        with pytest.raises(AtxError):
+            b.set_tuning(chunk_px=100)
+        from ataraxia_amd import _lib
+        buf = np.zeros(64, np.uint32)
+        assert _lib.lib.atx_batch_get_image(b._h, 5, buf.ctypes.data, buf.size) == _lib.ATX_ERR_NOTFOUND

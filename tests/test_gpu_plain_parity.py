@@ -12,11 +12,14 @@ import numpy as np
 import pytest
 
 import oracle as O
-from ataraxia_amd import ATX_ACCUM_KAHAN, Camera, Renderer, Settings, default_scene, plain_1080p_scene
+from ataraxia_amd import ATX_ACCUM_KAHAN, ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Camera, Renderer, Settings
+from ataraxia_amd import default_scene, plain_1080p_scene
 from ataraxia_amd import plain_cpu_scene
 from ataraxia_amd.scene import FlatScene, make_lights, make_materials, make_spheres, scene_from_camera_overrides
 
 pytestmark = pytest.mark.gpu
+
+KERNELS = pytest.mark.parametrize("kernel", [ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT], ids=["plain", "wavefront"])
 
 RAD_TOL = 1e-4
 
@@ -35,12 +38,12 @@ def _rel_err(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
 
 
-def run_parity(sc: FlatScene, frames: int, kahan: bool = False):
+def run_parity(sc: FlatScene, frames: int, kahan: bool = False, kernel: int = ATX_KERNEL_PLAIN):
     cam = sc.camera
     st = sc.settings
     nb = max(1, st.max_bounces)
     print(f"case {sc.name} {cam.width}x{cam.height} spheres={sc.num_spheres} lights={len(sc.lights)} "
-          f"bounces={st.max_bounces} frames={frames} kahan={kahan}", flush=True)
+          f"bounces={st.max_bounces} frames={frames} kahan={kahan} kernel={kernel}", flush=True)
     osc = _oracle_scene(sc)
     ocam = O.camera_from(cam)
     oset = O.settings(st.max_bounces, st.sky_light, st.accumulation)
@@ -48,6 +51,7 @@ def run_parity(sc: FlatScene, frames: int, kahan: bool = False):
     o_acc = np.zeros((n, 4), np.float32)
     o_comp = np.zeros((n, 4), np.float32) if kahan else None
     with Renderer() as r:
+        r.set_kernel(kernel)
         r.on_resize(cam.width, cam.height)
         r.set_settings(st)
         if kahan:
@@ -77,57 +81,65 @@ def run_parity(sc: FlatScene, frames: int, kahan: bool = False):
     return acc, o_acc, total_rays
 
 
-def test_default_scene_small():
+@KERNELS
+def test_default_scene_small(kernel):
     sc = scene_from_camera_overrides(default_scene(), 64, 36)
     sc.settings = Settings(True, False, 6)
-    acc, oacc, _ = run_parity(sc, 4)
+    acc, oacc, _ = run_parity(sc, 4, kernel=kernel)
     assert np.array_equal(acc, oacc)
 
 
-def test_default_scene_sky_light():
+@KERNELS
+def test_default_scene_sky_light(kernel):
     sc = scene_from_camera_overrides(default_scene(), 48, 40)
     sc.settings = Settings(True, True, 4)
-    run_parity(sc, 3)
+    run_parity(sc, 3, kernel=kernel)
 
 
-def test_plain_cpu_config_reduced():
+@KERNELS
+def test_plain_cpu_config_reduced(kernel):
     sc = scene_from_camera_overrides(plain_cpu_scene(), 160, 90)
-    acc, oacc, _ = run_parity(sc, 4)
+    acc, oacc, _ = run_parity(sc, 4, kernel=kernel)
     assert np.array_equal(acc, oacc)
 
 
-def test_plain_1080p_scene_reduced():
+@KERNELS
+def test_plain_1080p_scene_reduced(kernel):
     sc = scene_from_camera_overrides(plain_1080p_scene(), 192, 108)
-    run_parity(sc, 3)
+    run_parity(sc, 3, kernel=kernel)
 
 
-def test_kahan_accumulator():
+@KERNELS
+def test_kahan_accumulator(kernel):
     sc = scene_from_camera_overrides(plain_cpu_scene(), 96, 54)
-    acc, oacc, _ = run_parity(sc, 5, kahan=True)
+    acc, oacc, _ = run_parity(sc, 5, kahan=True, kernel=kernel)
     assert np.array_equal(acc, oacc)
 
 
-def test_odd_sizes_and_tiny_images():
+@KERNELS
+def test_odd_sizes_and_tiny_images(kernel):
     for w, h in [(1, 1), (17, 5), (33, 31)]:
         sc = scene_from_camera_overrides(plain_cpu_scene(), w, h)
-        run_parity(sc, 2)
+        run_parity(sc, 2, kernel=kernel)
 
 
-def test_no_lights_zero_bounces_empty_scene():
+@KERNELS
+def test_no_lights_zero_bounces_empty_scene(kernel):
     base = scene_from_camera_overrides(plain_cpu_scene(), 40, 24)
     nl = FlatScene(base.spheres, base.materials, make_lights(0), Settings(True, False, 5), base.camera, name="nolights")
-    run_parity(nl, 2)
+    run_parity(nl, 2, kernel=kernel)
     zb = FlatScene(base.spheres, base.materials, base.lights, Settings(True, False, 0), base.camera, name="zero")
-    acc, _, _ = run_parity(zb, 2)
+    acc, _, _ = run_parity(zb, 2, kernel=kernel)
     assert np.all(acc[:, :3] == 0)
     empty = FlatScene(make_spheres(0), make_materials(0), base.lights, Settings(True, True, 3), base.camera,
                       name="empty-sky")
-    acc, _, _ = run_parity(empty, 2)
+    acc, _, _ = run_parity(empty, 2, kernel=kernel)
     # a sky-only path returns exactly (0.6, 0.7, 0.9) every frame
     assert np.allclose(acc[:, :3] / 2.0, np.array([0.6, 0.7, 0.9], np.float32), rtol=0, atol=0)
 
 
-def test_emissive_metallic_and_clamped_ids():
+@KERNELS
+def test_emissive_metallic_and_clamped_ids(kernel):
     sc = scene_from_camera_overrides(plain_cpu_scene(), 64, 48)
     mat = sc.materials.copy()
     mat["emission_color"][1] = (1.0, 0.5, 0.25)
@@ -136,17 +148,18 @@ def test_emissive_metallic_and_clamped_ids():
     sph = sc.spheres.copy()
     sph["material"][3] = 99          # out of range: clamped to 0 on upload
     s2 = FlatScene(sph, mat, sc.lights, Settings(True, False, 5), sc.camera, name="emissive")
-    run_parity(s2, 3)
+    run_parity(s2, 3, kernel=kernel)
 
 
-def test_large_scene_global_memory_path():
+@KERNELS
+def test_large_scene_global_memory_path(kernel):
     """More spheres than the LDS budget (64 KiB = 4096 spheres): global loads."""
     from ataraxia_amd.scene import SplitMix64, random_scene
     rng = SplitMix64(77)
     sph, mat, lig = random_scene(rng, 4200, 3, 40.0, (0.0, 6.0), (10.0, 20.0))
     cam = Camera.looking_at((0.0, 16.0, 80.0), width=40, height=24)
     sc = FlatScene(sph, mat, lig, Settings(True, False, 4), cam, name="n4201")
-    run_parity(sc, 2)
+    run_parity(sc, 2, kernel=kernel)
 
 
 def test_accumulation_off_resets_every_frame():
