@@ -1,11 +1,12 @@
 // autotune.cpp -- picks the batch renderer's chunk size and persistent grid by
 // timing candidates on the batch itself, and caches the choice.
 //
-// Candidates: chunk_px in {64, 128, 256} x frames per launch in {8, 16, 32} x
+// Candidates: chunk_px in {64, 128, 256} x frames per launch in {16, 32} x
 // grid in {full, 3/4} of the device's resident-workgroup capacity.  Each
-// candidate renders the batch's first two launches twice (the second run is
-// timed with HIP events; the graph is re-captured per candidate) and the one
-// with the highest samples/ms wins.  The cache is a text file of
+// candidate renders the first F frames of a spread-out subsample of at most 8
+// of the batch's scenes twice (the second run is timed with HIP events; the
+// graph is re-captured per candidate) and the one with the highest samples/ms
+// wins.  The cache is a text file of
 // "<key>\t<chunk_px>\t<workgroups>\t<frames_per_launch>" lines; the key
 // names the device (arch, CU count) and the workload (scene count, pixels,
 // spheres, spp, bounces).
@@ -79,10 +80,16 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
         }
     }
 
-    // candidates: chunk size x frames per launch x grid fill; each renders the
-    // batch's first 2F frames twice (the second run is timed)
+    // candidates: chunk size x frames per launch x grid fill, each timed on a
+    // spread-out subsample of at most 8 scenes (their first F frames, run twice,
+    // the second run timed)
+    std::vector<uint32_t> sample;
+    const size_t total = batch_size(b);
+    const size_t step = total > 8 ? total / 8 : 1;
+    for (size_t i = 0; i < total && sample.size() < 8; i += step) sample.push_back((uint32_t)i);
+    batch_set_subset(b, sample);
     const uint32_t chunks[] = {64u, 128u, 256u};
-    const uint32_t frames[] = {8u, 16u, 32u};
+    const uint32_t frames[] = {16u, 32u};
     double best_rate = -1.0;
     atx_batch_tuning best{kDefaultChunkPx, 0, 1, kDefaultFramesPerLaunch};
     for (uint32_t c : chunks) {
@@ -95,10 +102,12 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
                 rc = atx_batch_set_tuning(h, &t);
                 if (rc != ATX_OK) return rc;
                 double ms = 0.0;
-                rc = batch_run(b, 2 * F, &ms);   // warm-up (also captures the graph)
-                if (rc != ATX_OK) return rc;
-                rc = batch_run(b, 2 * F, &ms);
-                if (rc != ATX_OK) return rc;
+                rc = batch_run(b, F, &ms);   // warm-up (also captures the graph)
+                if (rc == ATX_OK) rc = batch_run(b, F, &ms);
+                if (rc != ATX_OK) {
+                    batch_set_subset(b, {});
+                    return rc;
+                }
                 atx_render_stats st{};
                 atx_batch_get_stats(h, &st);
                 const double rate = ms > 0.0 ? (double)st.samples / ms : 0.0;   // samples per ms
@@ -109,6 +118,7 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
             }
         }
     }
+    batch_set_subset(b, {});
     rc = atx_batch_set_tuning(h, &best);
     if (rc != ATX_OK) return rc;
     if (chosen) *chosen = best;

@@ -51,6 +51,7 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
     uint32_t prepared_frames = 0;
+    std::vector<uint32_t> subset;          // autotuning: render only these scene ids (empty = all)
     atx_render_stats stats{};
 };
 
@@ -84,9 +85,13 @@ static int enqueue_launches(Batch* b)
 static int batch_prepare(Batch* b, uint32_t max_frames)
 {
     drop_graph(b);
-    const size_t n = b->scenes.size();
-    b->order.resize(n);
-    std::iota(b->order.begin(), b->order.end(), 0u);
+    if (b->subset.empty()) {
+        b->order.resize(b->scenes.size());
+        std::iota(b->order.begin(), b->order.end(), 0u);
+    } else {
+        b->order = b->subset;
+    }
+    const size_t n = b->order.size();
     std::stable_sort(b->order.begin(), b->order.end(),
                      [b](uint32_t x, uint32_t y) { return b->scenes[x]->spp > b->scenes[y]->spp; });
     b->chunk_px = b->tuning.chunk_px ? b->tuning.chunk_px : kDefaultChunkPx;
@@ -173,9 +178,10 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
     float ms = 0.0f;
     ATX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     uint64_t samples = 0;
-    for (auto& e : b->scenes) {
-        const uint64_t frames = std::min<uint64_t>(e->spp, b->prepared_frames);
-        samples += (uint64_t)e->cam.width * e->cam.height * frames;
+    for (uint32_t id : b->order) {
+        const BatchEntry& e = *b->scenes[id];
+        const uint64_t frames = std::min<uint64_t>(e.spp, b->prepared_frames);
+        samples += (uint64_t)e.cam.width * e.cam.height * frames;
     }
     b->stats.samples = samples;
     b->stats.rays = ctr[kCtrRays];
@@ -189,6 +195,14 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
 Batch* batch_cast(atx_batch* h) { return reinterpret_cast<Batch*>(h); }
 
 bool batch_is_kahan(const Batch* b) { return b->accumulator == ATX_ACCUM_KAHAN; }
+
+void batch_set_subset(Batch* b, const std::vector<uint32_t>& ids)
+{
+    b->subset = ids;
+    b->dirty = true;
+}
+
+size_t batch_size(const Batch* b) { return b->scenes.size(); }
 
 void batch_signature(const Batch* b, uint64_t* scenes, uint64_t* pixels, uint64_t* spheres, uint64_t* spp,
                      uint64_t* bounces)

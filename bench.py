@@ -1,17 +1,29 @@
-"""Benchmark: path samples/s of the 1080p workload on one MI355X.
+"""Benchmark of the MI355X path tracer (one GPU).
 
-Workload (the metric's configuration): 1920x1080, 256 random spheres + a
-ground sphere, 8 point lights, 8 bounces, 1 sample per pixel per frame,
-accumulated; synthetic scene from the seeded generator.  One "step" is one
-frame (one path sample for every pixel).
+Default workload (the metric's configuration), ``--workload plain_1080p``:
+1920x1080, 256 random spheres + a ground sphere, 8 point lights, 8 bounces,
+1 sample per pixel per frame, accumulated over the timed frames; synthetic
+scene from the seeded generator.  One "step" is one frame (one path sample
+for every pixel).  `value` is the single-scene renderer's production path
+(the persistent wavefront kernel, several frames per launch); the plain
+one-lane-per-pixel kernel is timed on the same frames and reported beside it.
+Both produce bit-identical images.
 
-Prints one JSON line with samples/s, the VALU roofline of the render kernel
-(17 algorithmic FLOP per ray-sphere test, measured with HIP events on the
-renderer's stream) and the CPU oracle timed on the host's cores.
+Other workloads (one JSON line each):
+  --workload batch_mix   64 mixed scenes: batched persistent renderer vs the
+                         plain kernel rendering the scenes one by one
+  --workload serve       Poisson request stream through the serving loop
+                         (samples/s, p50/p99 latency, HBM high-water mark)
+
+The roofline entry is the VALU bound of the dominant kernel: 17 algorithmic
+FLOP per ray-sphere test (FMA = 2), tests counted exactly by the kernels,
+divided by that kernel's average launch duration from HIP events recorded on
+the renderer's stream, against MI355X's published FP32 vector peak.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,11 +39,24 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256, help="timed frames")
     ap.add_argument("--warmup", type=int, default=8, help="untimed frames")
+    ap.add_argument("--workload", choices=["plain_1080p", "batch_mix", "serve"], default="plain_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time budget of the CPU baseline")
     ap.add_argument("--exact-loop", action="store_true", help="use the exact (slow) closest-hit loop")
-    ap.add_argument("--kernel", choices=["plain", "wavefront"], default="plain")
+    ap.add_argument("--kernel", choices=["plain", "wavefront", "both"], default="both")
+    ap.add_argument("--scenes", type=int, default=64, help="batch_mix: number of scenes")
+    ap.add_argument("--spp-scale", type=float, default=1.0, help="batch_mix: scale every scene's spp")
     return ap.parse_args()
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch of a kernel from a committed rocprof summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    mb = [float(m) for m in re.findall(r"= ([0-9.]+) MB \(calibrated\)", open(path).read())]
+    return sum(mb) * 1e6 if len(mb) >= 2 else None
 
 
 def cpu_baseline(sc, target_s):
@@ -60,38 +85,54 @@ def cpu_baseline(sc, target_s):
                       f"in {dt:.2f} s on {threads} threads"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    if args.exact_loop:
-        os.environ["ATX_EXACT_LOOP"] = "1"
-    from ataraxia_amd import ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Renderer, plain_1080p_scene
-
-    sc = plain_1080p_scene()
+def time_single_scene(sc, kernel, steps, warmup):
+    from ataraxia_amd import Renderer
+    from ataraxia_amd._lib import lib
     cam = sc.camera
     r = Renderer()
-    r.set_kernel(ATX_KERNEL_WAVEFRONT if args.kernel == "wavefront" else ATX_KERNEL_PLAIN)
+    r.set_kernel(kernel)
     r.on_resize(cam.width, cam.height)
     r.set_settings(sc.settings)
     r.set_scene(sc)
-    if args.warmup:
-        r.render_frames(cam, args.warmup)
+    if warmup:
+        r.render_frames(cam, warmup)
     r.reset_frame_index()
-    # timed region: K frames queued back to back, synchronised on both sides
-    from ataraxia_amd._lib import lib
     lib.atx_device_synchronize()
     t0 = time.perf_counter()
-    r.render_frames(cam, args.steps)
+    r.render_frames(cam, steps)
     lib.atx_device_synchronize()
     wall = time.perf_counter() - t0
     st = r.stats()
+    r.close()
+    return wall, st
+
+
+def roofline(st, launches, traffic):
+    kernel_ms = st["gpu_ms"] / launches
+    tests = st["sphere_tests"] / launches
+    achieved = FLOP_PER_TEST * tests / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": kernel_ms,
+            "launches": launches, "sphere_tests_per_launch": tests, "rays_per_launch": st["rays"] / launches,
+            "exact_retraces": st["exact_retraces"]}
+
+
+def bench_plain_1080p(args):
+    from ataraxia_amd import ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, plain_1080p_scene
+    sc = plain_1080p_scene()
+    cam = sc.camera
     samples = cam.width * cam.height * args.steps
-    kernel_ms = st["gpu_ms"] / args.steps
-    tests_per_launch = st["sphere_tests"] / args.steps
-    achieved = FLOP_PER_TEST * tests_per_launch / (kernel_ms * 1e-3) / 1e12
+    res = {}
+    if args.kernel in ("plain", "both"):
+        wall, st = time_single_scene(sc, ATX_KERNEL_PLAIN, args.steps, args.warmup)
+        res["plain"] = (wall, st, roofline(st, args.steps, traffic_from_profile("plain_1080p_k_render_plain.txt")))
+    if args.kernel in ("wavefront", "both"):
+        wall, st = time_single_scene(sc, ATX_KERNEL_WAVEFRONT, args.steps, args.warmup)
+        launches = max(1, -(-args.steps // int(os.environ.get("ATX_WF_FRAMES", "16"))))
+        res["wavefront"] = (wall, st, roofline(st, launches,
+                                               traffic_from_profile("plain_1080p_k_render_wavefront.txt")))
+    main_kernel = "wavefront" if "wavefront" in res else "plain"
+    wall, st, roof = res[main_kernel]
     out = {
         "metric": "path samples/s (1080p, 8 bounces)",
         "value": samples / wall,
@@ -106,17 +147,90 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded splitmix64 scene generator)",
         "config": {"workload": "plain_1080p: 1920x1080, 256 spheres + ground, 8 lights, 8 bounces, "
-                               "1 spp/frame accumulated", "kernel": args.kernel,
-                   "spheres": int(sc.num_spheres), "lights": int(len(sc.lights)), "frames": args.steps},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": kernel_ms, "rays_per_launch": st["rays"] / args.steps,
-                     "sphere_tests_per_launch": tests_per_launch,
-                     "exact_retraces": st["exact_retraces"]},
+                               "1 spp/frame accumulated", "renderer": "single-scene",
+                   "kernel": main_kernel, "spheres": int(sc.num_spheres), "lights": int(len(sc.lights)),
+                   "frames": args.steps},
+        "roofline": roof,
     }
+    if "plain" in res and main_kernel != "plain":
+        pw, pst, proof = res["plain"]
+        out["plain_kernel"] = {"value": samples / pw, "ms_per_step": pw * 1e3 / args.steps, "roofline": proof}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
+    return out
+
+
+def bench_batch_mix(args):
+    """Batched persistent renderer vs the plain kernel on the same 64 scenes."""
+    from ataraxia_amd import ATX_KERNEL_PLAIN, Batch, Renderer, batch_mix_scenes
+    from ataraxia_amd._lib import lib
+    scenes = batch_mix_scenes()[: args.scenes]
+    for s in scenes:
+        s.spp = max(1, int(round(s.spp * args.spp_scale)))
+    samples = sum(s.camera.width * s.camera.height * s.spp for s in scenes)
+    b = Batch()
+    for s in scenes:
+        b.add_scene(s)
+    t0 = time.perf_counter()
+    cache_dir = os.path.join(ROOT, "gpurun_out")
+    tuning = b.autotune(os.path.join(cache_dir, "atx_autotune.txt") if os.path.isdir(cache_dir) else None)
+    tune_s = time.perf_counter() - t0
+    b.render(max_frames=2)       # warm-up: graph capture + first frames
+    lib.atx_device_synchronize()
+    t0 = time.perf_counter()
+    bst = b.render()
+    lib.atx_device_synchronize()
+    bwall = time.perf_counter() - t0
+    b.close()
+    # plain kernel, scene after scene, one launch per frame
+    r = Renderer()
+    r.set_kernel(ATX_KERNEL_PLAIN)
+    pwall, ptests, pms = 0.0, 0, 0.0
+    for s in scenes:
+        r.on_resize(s.camera.width, s.camera.height)
+        r.set_settings(s.settings)
+        r.set_scene(s)
+        r.reset_frame_index()
+        lib.atx_device_synchronize()
+        t0 = time.perf_counter()
+        r.render_frames(s.camera, s.spp)
+        lib.atx_device_synchronize()
+        pwall += time.perf_counter() - t0
+        st = r.stats()
+        ptests += st["sphere_tests"]
+        pms += st["gpu_ms"]
     r.close()
+    bv, pv = samples / bwall, samples / pwall
+    return {
+        "metric": "path samples/s, batched vs plain on the mixed workload",
+        "value": bv, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": bwall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded splitmix64 scene generator)",
+        "config": {"workload": f"batch_mix: {len(scenes)} scenes, 256..1920 x 256..1080, 16..4096 spheres, "
+                               f"1..16 lights, 2..10 bounces, spp x{args.spp_scale}", "samples": samples,
+                   "tuning": tuning, "autotune_s": tune_s},
+        "batched": {"samples_per_s": bv, "wall_s": bwall, "gpu_ms": bst["gpu_ms"],
+                    "tflops": FLOP_PER_TEST * bst["sphere_tests"] / (bst["gpu_ms"] * 1e-3) / 1e12},
+        "plain": {"samples_per_s": pv, "wall_s": pwall, "gpu_ms": pms,
+                  "tflops": FLOP_PER_TEST * ptests / (pms * 1e-3) / 1e12},
+        "speedup": bv / pv,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.exact_loop:
+        os.environ["ATX_EXACT_LOOP"] = "1"
+    if args.workload == "plain_1080p":
+        out = bench_plain_1080p(args)
+    elif args.workload == "batch_mix":
+        out = bench_batch_mix(args)
+    else:
+        raise SystemExit("serve workload: not available in this build yet")
     print(json.dumps(out))
 
 
