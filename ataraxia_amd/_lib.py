@@ -35,6 +35,7 @@ ATX_ACCUM_NAIVE = 0
 ATX_ACCUM_KAHAN = 1
 ATX_KERNEL_PLAIN = 0
 ATX_KERNEL_WAVEFRONT = 1
+ATX_REQ_QUEUED, ATX_REQ_RUNNING, ATX_REQ_DONE, ATX_REQ_FAILED = 0, 1, 2, 3
 
 
 class Sphere(C.Structure):
@@ -71,6 +72,24 @@ class SceneDesc(C.Structure):
 class BatchTuning(C.Structure):
     _fields_ = [("chunk_px", C.c_uint32), ("workgroups", C.c_uint32), ("use_graph", C.c_uint32),
                 ("frames_per_launch", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("frames_per_tick", C.c_uint32), ("chunk_px", C.c_uint32),
+                ("max_active", C.c_uint32), ("kahan", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("rejected", C.c_uint64), ("failed", C.c_uint64), ("queued", C.c_uint64), ("active", C.c_uint64),
+                ("samples", C.c_uint64), ("ticks", C.c_uint64), ("hbm_in_use", C.c_uint64),
+                ("hbm_high_water", C.c_uint64), ("device_used_high_water", C.c_uint64),
+                ("gpu_busy_ms", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("frames_done", C.c_uint32), ("t_submit_s", C.c_double),
+                ("t_admit_s", C.c_double), ("t_done_s", C.c_double)]
 
 
 class RenderStats(C.Structure):
@@ -130,6 +149,13 @@ PROTOTYPES = {
     "atx_batch_get_image": (_I, [_P, _U32, _P, _U64]),
     "atx_batch_get_accumulation": (_I, [_P, _U32, _P, _U64]),
     "atx_batch_get_stats": (_I, [_P, C.POINTER(RenderStats)]),
+    "atx_server_create": (_I, [C.POINTER(_P), C.POINTER(ServerConfig)]),
+    "atx_server_destroy": (_I, [_P]),
+    "atx_server_submit": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(_U64)]),
+    "atx_server_get_info": (_I, [_P, _U64, C.POINTER(RequestInfo)]),
+    "atx_server_wait": (_I, [_P, _U64, C.c_double]),
+    "atx_server_get_result": (_I, [_P, _U64, _P, _U64, _P, _U64, _I]),
+    "atx_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
     "atx_camera_ray_directions": (_I, [C.POINTER(CameraDesc), _P, _U64]),
     "atx_camera_matrices": (_I, [C.POINTER(CameraDesc), _P, _P]),
 }

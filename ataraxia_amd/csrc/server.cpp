@@ -1,0 +1,428 @@
+// server.cpp -- continuous-batching render service on one GPU.
+//
+// Submitters copy their scene into a request and return at once.  One worker
+// thread owns the device: between launches it admits queued requests in FIFO
+// order while their device buffers fit the HBM budget (geometry, accumulation,
+// optional Kahan compensation, RGBA8 image; the head of the queue is never
+// overtaken), then renders every admitted request together with one launch of
+// the persistent wavefront kernel (frames_per_tick frames per request, each
+// request at its own frame index).  Requests that arrive while a launch runs
+// join at the next launch.  A request that reaches its spp is copied to host
+// memory and its device buffers are released immediately, which makes room
+// for the next admissions.  Per-request timestamps (submit, admit, done) give
+// the latency distribution.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "device_types.h"
+#include "host_common.h"
+#include "wavefront_host.h"
+
+namespace atx {
+
+struct Request {
+    uint64_t id = 0;
+    // host copy of the scene
+    std::vector<atx_sphere> spheres;
+    std::vector<atx_material> materials;
+    std::vector<atx_light> lights;
+    atx_settings settings{};
+    atx_camera cam{};
+    uint32_t spp = 1;
+    size_t device_bytes = 0;
+    // device state while admitted
+    DevScene geo;
+    DevBuf accum, comp, image;
+    uint32_t frames_done = 0;
+    // results
+    std::vector<uint32_t> rgba;
+    std::vector<float> acc_host;
+    int state = ATX_REQ_QUEUED;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    std::string error;
+};
+
+struct Server {
+    atx_server_config cfg{};
+    std::chrono::steady_clock::time_point epoch;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;          // waiting for admission
+    std::vector<std::shared_ptr<Request>> active;        // worker-owned
+    std::map<uint64_t, std::shared_ptr<Request>> all;    // every live request by id
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    atx_server_stats stats{};
+    // worker-only device resources
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf table, queue_ctr, counters;
+    uint32_t grid = 0;
+    int device = 0;
+};
+
+static double now_s(const Server* s)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->epoch).count();
+}
+
+static size_t request_bytes(const Request& r, bool kahan)
+{
+    const size_t npx = (size_t)r.cam.width * r.cam.height;
+    return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
+           npx * (16 + (kahan ? 16 : 0) + 4);
+}
+
+// Device allocations of one request (worker thread).
+static bool admit(Server* s, Request& r)
+{
+    int rc = upload_scene(r.spheres.data(), r.spheres.size(), r.materials.data(), r.materials.size(),
+                          r.lights.data(), r.lights.size(), &r.geo);
+    const size_t npx = (size_t)r.cam.width * r.cam.height;
+    if (rc == ATX_OK && (r.accum.reserve(npx * 16) != hipSuccess || r.image.reserve(npx * 4) != hipSuccess ||
+                         (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
+        rc = ATX_ERR_NOMEM;
+    if (rc != ATX_OK) {
+        r.error = atx_last_error();
+        r.geo = DevScene();
+        r.accum.release();
+        r.comp.release();
+        r.image.release();
+        return false;
+    }
+    return true;
+}
+
+static void finish(Server* s, Request& r, bool ok)
+{
+    const size_t npx = (size_t)r.cam.width * r.cam.height;
+    if (ok) {
+        r.rgba.resize(npx);
+        r.acc_host.resize(4 * npx);
+        if (hipMemcpy(r.rgba.data(), r.image.get(), npx * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(r.acc_host.data(), r.accum.get(), npx * 16, hipMemcpyDeviceToHost) != hipSuccess) {
+            ok = false;
+            r.error = "result copy failed";
+        }
+    }
+    r.geo = DevScene();
+    r.accum.release();
+    r.comp.release();
+    r.image.release();
+    std::lock_guard<std::mutex> lk(s->mu);
+    r.state = ok ? ATX_REQ_DONE : ATX_REQ_FAILED;
+    r.t_done = now_s(s);
+    s->stats.hbm_in_use -= r.device_bytes;
+    if (ok) {
+        s->stats.completed++;
+        s->stats.samples += (uint64_t)npx * r.spp;
+    } else {
+        s->stats.failed++;
+    }
+    s->cv_done.notify_all();
+}
+
+static void worker_main(Server* s)
+{
+    (void)hipSetDevice(s->device);
+    std::vector<WfScene> table;
+    for (;;) {
+        // ---- admission (FIFO, HBM budget) ----
+        std::vector<std::shared_ptr<Request>> newly;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty() || !s->active.empty(); });
+            if (s->stop) break;
+            while (!s->queue.empty()) {
+                auto& r = s->queue.front();
+                if (s->cfg.max_active && s->active.size() + newly.size() >= s->cfg.max_active) break;
+                if (s->stats.hbm_in_use + r->device_bytes > s->cfg.hbm_budget_bytes) break;
+                s->stats.hbm_in_use += r->device_bytes;
+                s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
+                newly.push_back(r);
+                s->queue.pop_front();
+            }
+        }
+        for (auto& r : newly) {
+            if (admit(s, *r)) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r->state = ATX_REQ_RUNNING;
+                r->t_admit = now_s(s);
+                s->stats.admitted++;
+                s->active.push_back(r);
+            } else {
+                finish(s, *r, false);
+            }
+        }
+        if (s->active.empty()) continue;
+
+        // ---- one launch over every admitted request ----
+        const uint32_t F = s->cfg.frames_per_tick;
+        const uint32_t P = s->cfg.chunk_px;
+        table.clear();
+        uint32_t chunks = 0;
+        for (auto& r : s->active) {
+            WfScene w;
+            fill_wf_scene(r->geo, r->settings, r->cam, r->accum.as<float4>(),
+                          s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(),
+                          s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done), &w);
+            w.chunk_begin = chunks;
+            w.n_chunks = chunks_for(r->cam.width * r->cam.height, P);
+            chunks += w.n_chunks;
+            table.push_back(w);
+        }
+        bool ok = s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
+                  hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene),
+                                 hipMemcpyHostToDevice, s->stream) == hipSuccess &&
+                  launch_zero_u32(s->queue_ctr.as<unsigned int>(), 1, s->stream) == hipSuccess &&
+                  hipEventRecord(s->ev0, s->stream) == hipSuccess;
+        if (ok) {
+            WfPass wp{};
+            wp.scenes = s->table.as<WfScene>();
+            wp.n_scenes = (uint32_t)table.size();
+            wp.total_chunks = chunks;
+            wp.chunk_px = P;
+            wp.kahan = s->cfg.kahan;
+            wp.queue = s->queue_ctr.as<unsigned int>();
+            wp.counters = s->counters.as<unsigned long long>();
+            ok = launch_wavefront(wp, WfLaunch{s->grid}, s->stream) == hipSuccess &&
+                 hipEventRecord(s->ev1, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
+        }
+        float ms = 0.0f;
+        if (ok) (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stats.ticks++;
+            s->stats.gpu_busy_ms += ms;
+            s->stats.device_used_high_water = std::max<uint64_t>(s->stats.device_used_high_water, total_b - free_b);
+        }
+        // ---- advance, retire finished requests ----
+        std::vector<std::shared_ptr<Request>> still;
+        for (auto& r : s->active) {
+            if (!ok) {
+                r->error = "launch failed";
+                finish(s, *r, false);
+                continue;
+            }
+            bool complete;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r->frames_done += std::min(F, r->spp - r->frames_done);
+                complete = r->frames_done >= r->spp;
+            }
+            if (complete) finish(s, *r, true);
+            else still.push_back(r);
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->active.swap(still);
+    }
+}
+
+static Server* cast(atx_server* h) { return reinterpret_cast<Server*>(h); }
+
+}  // namespace atx
+
+using namespace atx;
+
+extern "C" {
+
+int atx_server_create(atx_server** out, const atx_server_config* cfg)
+{
+    ATX_CHECK_ARG(out, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device available");
+        return ATX_ERR_NODEVICE;
+    }
+    std::unique_ptr<Server> s(new (std::nothrow) Server());
+    if (!s) { set_error("out of host memory"); return ATX_ERR_NOMEM; }
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = 8;
+    if (!s->cfg.chunk_px) s->cfg.chunk_px = kDefaultChunkPx;
+    ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be 64..256, step 64");
+    ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");
+    s->cfg.kahan = s->cfg.kahan ? 1u : 0u;
+    ATX_HIP(hipGetDevice(&s->device));
+    if (!s->cfg.hbm_budget_bytes) {
+        size_t free_b = 0, total_b = 0;
+        ATX_HIP(hipMemGetInfo(&free_b, &total_b));
+        s->cfg.hbm_budget_bytes = free_b / 2;
+    }
+    ATX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    ATX_HIP(hipEventCreate(&s->ev0));
+    ATX_HIP(hipEventCreate(&s->ev1));
+    ATX_HIP(s->queue_ctr.reserve(256));
+    ATX_HIP(s->counters.reserve(kCtrCount * sizeof(unsigned long long)));
+    ATX_HIP(hipMemset(s->counters.get(), 0, kCtrCount * sizeof(unsigned long long)));
+    s->grid = wavefront_full_grid(s->cfg.chunk_px, s->cfg.kahan != 0);
+    s->epoch = std::chrono::steady_clock::now();
+    Server* raw = s.release();
+    try {
+        raw->worker = std::thread(worker_main, raw);
+    } catch (...) {
+        set_error("cannot start the server thread");
+        delete raw;
+        return ATX_ERR_STATE;
+    }
+    *out = reinterpret_cast<atx_server*>(raw);
+    return ATX_OK;
+}
+
+int atx_server_destroy(atx_server* h)
+{
+    Server* s = cast(h);
+    if (!s) return ATX_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->active.clear();
+    s->queue.clear();
+    s->all.clear();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return ATX_OK;
+}
+
+int atx_server_submit(atx_server* h, const atx_scene_desc* d, uint64_t* request_id)
+{
+    Server* s = cast(h);
+    ATX_CHECK_ARG(s && d, "NULL argument");
+    ATX_CHECK_ARG(d->camera.width > 0 && d->camera.height > 0, "Width or height cannot be zero");
+    ATX_CHECK_ARG((uint64_t)d->camera.width * d->camera.height < (1ull << 31), "viewport too large");
+    ATX_CHECK_ARG(d->settings.max_bounces >= 0, "max_bounces must be >= 0");
+    ATX_CHECK_ARG(d->spp >= 1, "spp must be >= 1");
+    ATX_CHECK_ARG(d->num_spheres == 0 || d->spheres, "spheres is NULL");
+    ATX_CHECK_ARG(d->num_materials == 0 || d->materials, "materials is NULL");
+    ATX_CHECK_ARG(d->num_lights == 0 || d->lights, "lights is NULL");
+    if (d->num_spheres > 0 && d->num_materials == 0) {
+        set_error("scene has spheres but no material");
+        return ATX_ERR_INVALID;
+    }
+    auto r = std::make_shared<Request>();
+    r->spheres.assign(d->spheres, d->spheres + d->num_spheres);
+    r->materials.assign(d->materials, d->materials + d->num_materials);
+    r->lights.assign(d->lights, d->lights + d->num_lights);
+    r->settings = d->settings;
+    r->cam = d->camera;
+    r->spp = d->spp;
+    r->device_bytes = request_bytes(*r, s->cfg.kahan != 0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.submitted++;
+    if (r->device_bytes > s->cfg.hbm_budget_bytes) {
+        s->stats.rejected++;
+        set_error("request needs %zu device bytes, budget is %llu", r->device_bytes,
+                  (unsigned long long)s->cfg.hbm_budget_bytes);
+        return ATX_ERR_BUDGET;
+    }
+    r->id = s->next_id++;
+    r->t_submit = now_s(s);
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+    if (request_id) *request_id = r->id;
+    s->cv_work.notify_one();
+    return ATX_OK;
+}
+
+int atx_server_get_info(atx_server* h, uint64_t id, atx_request_info* info)
+{
+    Server* s = cast(h);
+    ATX_CHECK_ARG(s && info, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        set_error("unknown request %llu", (unsigned long long)id);
+        return ATX_ERR_NOTFOUND;
+    }
+    const Request& r = *it->second;
+    info->state = r.state;
+    info->frames_done = r.frames_done;
+    info->t_submit_s = r.t_submit;
+    info->t_admit_s = r.t_admit;
+    info->t_done_s = r.t_done;
+    return ATX_OK;
+}
+
+int atx_server_wait(atx_server* h, uint64_t id, double timeout_s)
+{
+    Server* s = cast(h);
+    ATX_CHECK_ARG(s, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        set_error("unknown request %llu", (unsigned long long)id);
+        return ATX_ERR_NOTFOUND;
+    }
+    std::shared_ptr<Request> r = it->second;
+    auto done = [&r] { return r->state == ATX_REQ_DONE || r->state == ATX_REQ_FAILED; };
+    if (timeout_s < 0) {
+        s->cv_done.wait(lk, done);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+        set_error("timed out waiting for request %llu", (unsigned long long)id);
+        return ATX_ERR_STATE;
+    }
+    if (r->state == ATX_REQ_FAILED) {
+        set_error("request %llu failed: %s", (unsigned long long)id, r->error.c_str());
+        return ATX_ERR_STATE;
+    }
+    return ATX_OK;
+}
+
+int atx_server_get_result(atx_server* h, uint64_t id, uint32_t* rgba, uint64_t num_pixels, float* accum,
+                          uint64_t num_floats, int release)
+{
+    Server* s = cast(h);
+    ATX_CHECK_ARG(s, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        set_error("unknown request %llu", (unsigned long long)id);
+        return ATX_ERR_NOTFOUND;
+    }
+    Request& r = *it->second;
+    if (r.state != ATX_REQ_DONE) {
+        set_error("request %llu is not done", (unsigned long long)id);
+        return ATX_ERR_STATE;
+    }
+    if (rgba) {
+        ATX_CHECK_ARG(num_pixels >= r.rgba.size(), "rgba buffer too small");
+        std::copy(r.rgba.begin(), r.rgba.end(), rgba);
+    }
+    if (accum) {
+        ATX_CHECK_ARG(num_floats >= r.acc_host.size(), "accum buffer too small");
+        std::copy(r.acc_host.begin(), r.acc_host.end(), accum);
+    }
+    if (release) s->all.erase(it);
+    return ATX_OK;
+}
+
+int atx_server_get_stats(atx_server* h, atx_server_stats* st)
+{
+    Server* s = cast(h);
+    ATX_CHECK_ARG(s && st, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    st->queued = s->queue.size();
+    st->active = s->active.size();
+    return ATX_OK;
+}
+
+}  // extern "C"

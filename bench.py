@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--kernel", choices=["plain", "wavefront", "both"], default="both")
     ap.add_argument("--scenes", type=int, default=64, help="batch_mix: number of scenes")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="batch_mix: scale every scene's spp")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
+    ap.add_argument("--frames-per-tick", type=int, default=8, help="serve: frames per request per launch")
+    ap.add_argument("--no-serve", action="store_true", help="plain_1080p: skip the short serving run")
     return ap.parse_args()
 
 
@@ -134,7 +138,7 @@ def bench_plain_1080p(args):
     main_kernel = "wavefront" if "wavefront" in res else "plain"
     wall, st, roof = res[main_kernel]
     out = {
-        "metric": "path samples/s (1080p, 8 bounces)",
+        "metric": "path samples/s (1080p, 8 bounces) + p50/p99 request latency in serve loop",
         "value": samples / wall,
         "unit": "path samples/s",
         "n_gpus": 1,
@@ -155,6 +159,8 @@ def bench_plain_1080p(args):
     if "plain" in res and main_kernel != "plain":
         pw, pst, proof = res["plain"]
         out["plain_kernel"] = {"value": samples / pw, "ms_per_step": pw * 1e3 / args.steps, "roofline": proof}
+    if not args.no_serve:
+        out["serve"] = bench_serve(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     return out
@@ -217,6 +223,46 @@ def bench_batch_mix(args):
     }
 
 
+def bench_serve(args, rate=None, requests=None):
+    """Poisson request stream through the continuous-batching server."""
+    import math
+    import numpy as np
+    from ataraxia_amd import Server, SplitMix64, serve_request_scene
+    rate = rate or args.rate
+    requests = requests or args.requests
+    rng = SplitMix64(4)
+    scenes = [serve_request_scene(rng, i) for i in range(requests)]
+    arr = SplitMix64(104)
+    t, arrivals = 0.0, []
+    for _ in scenes:
+        t += -math.log(1.0 - arr.uniform()) / rate
+        arrivals.append(t)
+    srv = Server(frames_per_tick=args.frames_per_tick)
+    t0 = time.perf_counter()
+    ids = []
+    for when, sc in zip(arrivals, scenes):
+        delay = t0 + when - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(sc))
+    for rid in ids:
+        srv.wait(rid, 600.0)
+    wall = time.perf_counter() - t0
+    infos = [srv.info(rid) for rid in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+    samples = sum(s.camera.width * s.camera.height * s.spp for s in scenes)
+    return {"rate_per_s": rate, "requests": requests, "samples": samples, "samples_per_s": samples / span,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "wall_s": wall, "hbm_high_water_bytes": st["hbm_high_water"],
+            "device_used_high_water_bytes": st["device_used_high_water"], "ticks": st["ticks"],
+            "gpu_busy_ms": st["gpu_busy_ms"], "frames_per_tick": args.frames_per_tick,
+            "requests_spec": "resolution from {512x512, 1280x720, 1920x1080}, spheres logU[4,1024] + ground, "
+                             "spp logU[16,1024], 8 bounces, 1..8 lights"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -230,7 +276,13 @@ def main():
     elif args.workload == "batch_mix":
         out = bench_batch_mix(args)
     else:
-        raise SystemExit("serve workload: not available in this build yet")
+        sv = bench_serve(args)
+        out = {"metric": "path samples/s + p50/p99 request latency in serve loop", "value": sv["samples_per_s"],
+               "unit": "path samples/s", "n_gpus": 1, "steps": sv["requests"], "warmup": 0,
+               "ms_per_step": sv["wall_s"] * 1e3 / sv["requests"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded splitmix64 request generator)",
+               "config": {"workload": f"serve: Poisson {sv['rate_per_s']}/s, {sv['requests']} requests"},
+               "serve": sv}
     print(json.dumps(out))
 
 

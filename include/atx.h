@@ -186,6 +186,60 @@ int atx_batch_get_image(atx_batch* b, uint32_t scene_id, uint32_t* rgba, uint64_
 int atx_batch_get_accumulation(atx_batch* b, uint32_t scene_id, float* rgba, uint64_t num_floats);
 int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 
+/* ---- Server: continuous-batching render service ----------------------------
+ * No reference counterpart.  Requests (a scene + spp) are admitted in FIFO
+ * order while their device buffers fit the HBM budget; every admitted request
+ * stays device-resident until it completes.  A worker thread renders all
+ * admitted requests together, frames_per_tick frames per request per launch of
+ * the persistent kernel, and admits newly arrived requests between launches.
+ * A finished request's image and accumulation are copied to host memory and
+ * its device buffers are released at once.                                  */
+#define ATX_REQ_QUEUED 0
+#define ATX_REQ_RUNNING 1
+#define ATX_REQ_DONE 2
+#define ATX_REQ_FAILED 3
+
+typedef struct atx_server_config {
+    uint64_t hbm_budget_bytes;  /* device bytes for request buffers (0 = half of free HBM) */
+    uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 8)          */
+    uint32_t chunk_px;          /* pixels per work chunk (0 = 128)                          */
+    uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
+    uint32_t kahan;             /* 1 = compensated accumulation                             */
+} atx_server_config;
+
+typedef struct atx_server_stats {
+    uint64_t submitted, admitted, completed, rejected, failed;
+    uint64_t queued, active;
+    uint64_t samples;              /* path samples of completed requests        */
+    uint64_t ticks;                /* launches issued                           */
+    uint64_t hbm_in_use;           /* bytes of the budget currently allocated   */
+    uint64_t hbm_high_water;       /* max of hbm_in_use                         */
+    uint64_t device_used_high_water; /* max device memory in use (hipMemGetInfo) */
+    double gpu_busy_ms;            /* HIP-event time of all launches            */
+} atx_server_stats;
+
+typedef struct atx_request_info {
+    int32_t state;                 /* ATX_REQ_*                                 */
+    uint32_t frames_done;
+    double t_submit_s, t_admit_s, t_done_s;  /* seconds since server creation   */
+} atx_request_info;
+
+typedef struct atx_server atx_server;
+
+int atx_server_create(atx_server** out, const atx_server_config* cfg);
+/* Stops the worker after the launch in flight and frees every buffer.       */
+int atx_server_destroy(atx_server* s);
+/* Copies the request; ATX_ERR_BUDGET if it could never fit the budget.      */
+int atx_server_submit(atx_server* s, const atx_scene_desc* req, uint64_t* request_id);
+int atx_server_get_info(atx_server* s, uint64_t request_id, atx_request_info* info);
+/* Waits until the request is DONE or FAILED (timeout_s < 0: no limit).       */
+int atx_server_wait(atx_server* s, uint64_t request_id, double timeout_s);
+/* Copies a finished request's RGBA8 image and/or accumulation (either may be
+ * NULL); release != 0 drops the host copy afterwards.                        */
+int atx_server_get_result(atx_server* s, uint64_t request_id, uint32_t* rgba, uint64_t num_pixels,
+                          float* accum, uint64_t num_floats, int release);
+int atx_server_get_stats(atx_server* s, atx_server_stats* stats);
+
 /* ---- Camera: host-side ray directions (Camera::UpdateRayDirection) -------- */
 /* Writes width*height*3 floats, bit-identical to what the kernels generate. */
 int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_floats);

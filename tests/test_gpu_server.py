@@ -1,0 +1,81 @@
+"""GPU tests of the continuous-batching server.
+
+Each request's result must equal the oracle's frame-by-frame accumulation and
+the result of serving that request alone, whatever else shares the device,
+whenever it arrives, and however the HBM budget throttles admission.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from _scenes import mixed_scenes, oracle_accumulate
+from ataraxia_amd import AtxError, Server
+from ataraxia_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def serve_all(scenes, delays=None, **cfg):
+    with Server(**cfg) as srv:
+        ids = []
+        for i, s in enumerate(scenes):
+            if delays:
+                time.sleep(delays[i])
+            ids.append(srv.submit(s))
+        out = []
+        for rid in ids:
+            srv.wait(rid, 60.0)
+            img, acc = srv.result(rid, release=True)
+            out.append((img.reshape(-1).copy(), acc.reshape(-1, 4).copy()))
+        st = srv.stats()
+    return out, st
+
+
+def test_server_matches_oracle_and_alone():
+    scenes = mixed_scenes()
+    print("case server mixed", [s.name for s in scenes], flush=True)
+    res, st = serve_all(scenes, frames_per_tick=2)
+    assert st["completed"] == len(scenes) and st["failed"] == 0
+    assert st["samples"] == sum(s.camera.width * s.camera.height * s.spp for s in scenes)
+    for (img, acc), s in zip(res, scenes):
+        oacc, oimg = oracle_accumulate(s, s.spp)
+        assert np.array_equal(acc, oacc), s.name
+        assert np.array_equal(img, oimg), s.name
+    for k in (0, 2):
+        alone, _ = serve_all([scenes[k]], frames_per_tick=3)
+        assert np.array_equal(alone[0][1], res[k][1])
+
+
+def test_server_staggered_arrivals_and_budget():
+    scenes = mixed_scenes()
+    ref, _ = serve_all(scenes, frames_per_tick=1)
+    # admit roughly one request at a time: budget just above the largest request
+    npx = max(s.camera.width * s.camera.height for s in scenes)
+    budget = npx * 20 + 4096 * 64 + (1 << 16)
+    res, st = serve_all(scenes, delays=[0.0, 0.01, 0.0, 0.02, 0.0, 0.01], frames_per_tick=1,
+                        hbm_budget_bytes=budget)
+    assert st["hbm_high_water"] <= budget
+    for (img, acc), (rimg, racc), s in zip(res, ref, scenes):
+        assert np.array_equal(acc, racc), s.name
+        assert np.array_equal(img, rimg), s.name
+
+
+def test_server_kahan_matches_oracle():
+    scenes = mixed_scenes()[:3]
+    res, _ = serve_all(scenes, frames_per_tick=2, kahan=True)
+    for (img, acc), s in zip(res, scenes):
+        oacc, _ = oracle_accumulate(s, s.spp, kahan=True)
+        assert np.array_equal(acc, oacc), s.name
+
+
+def test_server_rejects_oversized_request_and_reports_errors():
+    scenes = mixed_scenes()
+    with Server(hbm_budget_bytes=1 << 12) as srv:
+        with pytest.raises(AtxError) as e:
+            srv.submit(scenes[1])
+        assert e.value.code == _lib.ATX_ERR_BUDGET
+        st = srv.stats()
+        assert st["rejected"] == 1 and st["submitted"] == 1
+        with pytest.raises(AtxError):
+            srv.info(12345)
