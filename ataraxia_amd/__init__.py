@@ -8,12 +8,13 @@ from ._lib import (ATX_ACCUM_KAHAN, ATX_ACCUM_NAIVE, ATX_KERNEL_PLAIN, ATX_KERNE
                    device_count, last_error)
 from .batch import Batch
 from .renderer import Renderer, camera_matrices, camera_ray_directions
+from .scene_graph import SceneGraph, write_ppm
 from .server import Server
 from .scene import (Camera, FlatScene, Settings, SplitMix64, batch_mix_scenes, default_scene,
                     plain_1080p_scene, plain_cpu_scene, precision_scene, serve_request_scene)
 
 __all__ = [
-    "ATX_ACCUM_KAHAN", "ATX_ACCUM_NAIVE", "ATX_KERNEL_PLAIN", "ATX_KERNEL_WAVEFRONT", "AtxError", "Batch", "Server", "LIB_PATH", "device_count", "last_error",
+    "ATX_ACCUM_KAHAN", "ATX_ACCUM_NAIVE", "ATX_KERNEL_PLAIN", "ATX_KERNEL_WAVEFRONT", "AtxError", "Batch", "SceneGraph", "Server", "write_ppm", "LIB_PATH", "device_count", "last_error",
     "Renderer", "camera_matrices", "camera_ray_directions", "Camera", "FlatScene", "Settings",
     "SplitMix64", "batch_mix_scenes", "default_scene", "plain_1080p_scene", "plain_cpu_scene",
     "precision_scene", "serve_request_scene",

@@ -156,6 +156,26 @@ PROTOTYPES = {
     "atx_server_wait": (_I, [_P, _U64, C.c_double]),
     "atx_server_get_result": (_I, [_P, _U64, _P, _U64, _P, _U64, _I]),
     "atx_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
+    "atx_scene_create": (_I, [C.POINTER(_P)]),
+    "atx_scene_destroy": (_I, [_P]),
+    "atx_scene_load_json": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "atx_scene_parse_json": (_I, [C.c_char_p, _U64, C.POINTER(_P)]),
+    "atx_scene_save_json": (_I, [_P, C.c_char_p]),
+    "atx_scene_to_json": (_I, [_P, C.c_char_p, _U64, C.POINTER(_U64)]),
+    "atx_scene_add_node": (_I, [_P, _U32, C.c_char_p, _P, _P, _P, C.POINTER(_U32)]),
+    "atx_scene_set_node_transform": (_I, [_P, _U32, _P, _P, _P]),
+    "atx_scene_add_sphere": (_I, [_P, _U32, C.POINTER(Sphere)]),
+    "atx_scene_add_material": (_I, [_P, C.POINTER(Material)]),
+    "atx_scene_add_light": (_I, [_P, C.POINTER(Light)]),
+    "atx_scene_set_settings": (_I, [_P, C.POINTER(Settings)]),
+    "atx_scene_get_settings": (_I, [_P, C.POINTER(Settings)]),
+    "atx_scene_set_camera": (_I, [_P, _P, _P, C.c_float]),
+    "atx_scene_get_camera": (_I, [_P, _P, _P, C.POINTER(C.c_float)]),
+    "atx_scene_counts": (_I, [_P, C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64)]),
+    "atx_scene_flatten": (_I, [_P, _P, _U64, C.POINTER(_U64)]),
+    "atx_scene_get_materials": (_I, [_P, _P, _U64, C.POINTER(_U64)]),
+    "atx_scene_get_lights": (_I, [_P, _P, _U64, C.POINTER(_U64)]),
+    "atx_write_ppm": (_I, [C.c_char_p, _P, _U32, _U32, _I]),
     "atx_camera_ray_directions": (_I, [C.POINTER(CameraDesc), _P, _U64]),
     "atx_camera_matrices": (_I, [C.POINTER(CameraDesc), _P, _P]),
 }

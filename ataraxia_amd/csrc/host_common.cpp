@@ -148,4 +148,33 @@ int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_f
     return ATX_OK;
 }
 
+int atx_write_ppm(const char* path, const uint32_t* rgba, uint32_t width, uint32_t height, int flip_vertical)
+{
+    ATX_CHECK_ARG(path && rgba, "NULL argument");
+    ATX_CHECK_ARG(width > 0 && height > 0, "Width or height cannot be zero");
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open %s for writing", path);
+        return ATX_ERR_IO;
+    }
+    fprintf(f, "P6\n%u %u\n255\n", width, height);
+    std::vector<unsigned char> row((size_t)width * 3);
+    bool ok = true;
+    for (uint32_t r = 0; r < height && ok; ++r) {
+        const uint32_t y = flip_vertical ? height - 1 - r : r;
+        const uint32_t* src = rgba + (size_t)y * width;
+        for (uint32_t x = 0; x < width; ++x) {
+            row[3 * x + 0] = (unsigned char)(src[x] & 0xFF);
+            row[3 * x + 1] = (unsigned char)((src[x] >> 8) & 0xFF);
+            row[3 * x + 2] = (unsigned char)((src[x] >> 16) & 0xFF);
+        }
+        ok = fwrite(row.data(), 1, row.size(), f) == row.size();
+    }
+    if (fclose(f) != 0 || !ok) {
+        set_error("write to %s failed", path);
+        return ATX_ERR_IO;
+    }
+    return ATX_OK;
+}
+
 }  // extern "C"

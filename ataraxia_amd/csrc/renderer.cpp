@@ -29,6 +29,7 @@ struct Renderer {
     bool has_scene = false;
     int kernel = ATX_KERNEL_PLAIN;
     DevBuf wf_table, wf_queue;
+    DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = kDefaultChunkPx;
     uint32_t wf_frames = kDefaultFramesPerLaunch;
     DevBuf accum, comp, image, counters;
@@ -79,7 +80,9 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     a.counters = r->counters.as<unsigned long long>();
 
     const size_t npx = (size_t)r->width * r->height;
-    DevBuf d_rad, d_hit, d_sh;
+    DevBuf& d_rad = r->dbg_rad;
+    DevBuf& d_hit = r->dbg_hit;
+    DevBuf& d_sh = r->dbg_sh;
     uint32_t variant = 0;
     if ((size_t)r->geo.n_sph * sizeof(float4) <= r->lds_budget) variant |= kVarLds;
     if (r->accumulator == ATX_ACCUM_KAHAN) variant |= kVarKahan;

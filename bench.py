@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256, help="timed frames")
     ap.add_argument("--warmup", type=int, default=8, help="untimed frames")
-    ap.add_argument("--workload", choices=["plain_1080p", "batch_mix", "serve"], default="plain_1080p")
+    ap.add_argument("--workload", choices=["plain_1080p", "batch_mix", "serve", "precision"], default="plain_1080p")
+    ap.add_argument("--precision-size", type=int, default=128, help="precision: image side in pixels")
+    ap.add_argument("--precision-spp", type=int, default=4096, help="precision: samples per pixel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time budget of the CPU baseline")
     ap.add_argument("--exact-loop", action="store_true", help="use the exact (slow) closest-hit loop")
@@ -263,6 +265,46 @@ def bench_serve(args, rate=None, requests=None):
                              "spp logU[16,1024], 8 bounces, 1..8 lights"}
 
 
+def bench_precision(args):
+    """Naive vs Kahan FP32 accumulation against a float64 host sum of the
+    identical per-sample radiance stream (captured from the GPU every frame)."""
+    import numpy as np
+    from ataraxia_amd import ATX_ACCUM_KAHAN, Renderer, precision_scene
+    from ataraxia_amd.scene import scene_from_camera_overrides
+    side, spp = args.precision_size, args.precision_spp
+    sc = scene_from_camera_overrides(precision_scene(), side, side)
+    cam = sc.camera
+    n = side * side
+    s64 = np.zeros((n, 4), np.float64)
+    s32 = np.zeros((n, 4), np.float32)
+    r = Renderer()
+    r.on_resize(side, side)
+    r.set_settings(sc.settings)
+    r.set_scene(sc)
+    r.set_accumulator(ATX_ACCUM_KAHAN)
+    t0 = time.perf_counter()
+    for _ in range(spp):
+        rad, _, _ = r.render_debug(cam, 1)
+        s64 += rad
+        s32 = s32 + rad
+    wall = time.perf_counter() - t0
+    kah = r.get_accumulation().reshape(n, 4)
+    r.close()
+
+    def err(x):
+        m = s64[:, :3] > 0
+        e = np.abs(x[:, :3].astype(np.float64)[m] - s64[:, :3][m]) / np.maximum(s64[:, :3][m], 1e-6)
+        return {"max": float(e.max()), "mean": float(e.mean())}
+    ek, en = err(kah), err(s32)
+    return {"metric": "accumulator relative error vs float64 at 4096 spp", "value": ek["max"],
+            "unit": "relative error (max over pixels and channels)", "n_gpus": 1, "steps": spp, "warmup": 0,
+            "ms_per_step": wall * 1e3 / spp, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded splitmix64 scene generator)",
+            "config": {"workload": f"precision: {side}x{side} (of 1024x1024), 512 spheres + ground, 4 lights, "
+                                   f"6 bounces, {spp} spp"},
+            "kahan": ek, "naive": en, "kahan_below_1e-6": ek["max"] < 1e-6}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -275,6 +317,8 @@ def main():
         out = bench_plain_1080p(args)
     elif args.workload == "batch_mix":
         out = bench_batch_mix(args)
+    elif args.workload == "precision":
+        out = bench_precision(args)
     else:
         sv = bench_serve(args)
         out = {"metric": "path samples/s + p50/p99 request latency in serve loop", "value": sv["samples_per_s"],

@@ -240,6 +240,49 @@ int atx_server_get_result(atx_server* s, uint64_t request_id, uint32_t* rgba, ui
                           float* accum, uint64_t num_floats, int release);
 int atx_server_get_stats(atx_server* s, atx_server_stats* stats);
 
+/* ---- Scene graph and scene.json (Scene.h:58-80, SceneNode.h:23-66) --------
+ * A host-side scene: a tree of nodes with translate * rotate(quaternion) *
+ * scale transforms holding spheres, plus materials, lights, settings and the
+ * camera's position, direction and fov.  Node 0 is the root ("Scene").      */
+typedef struct atx_scene_graph atx_scene_graph;
+
+int atx_scene_create(atx_scene_graph** out);
+int atx_scene_destroy(atx_scene_graph* g);
+/* Utils::importScene (Utils.cpp:175-187): a file that cannot be opened gives
+ * an empty default scene (ATX_OK); malformed JSON gives ATX_ERR_IO.          */
+int atx_scene_load_json(const char* path, atx_scene_graph** out);
+int atx_scene_parse_json(const char* text, uint64_t len, atx_scene_graph** out);
+/* Utils::exportScene (Utils.cpp:53-62): 4-space indented JSON, sorted keys. */
+int atx_scene_save_json(const atx_scene_graph* g, const char* path);
+/* Writes the JSON text (NUL-terminated) when buf != NULL; *len = its length. */
+int atx_scene_to_json(const atx_scene_graph* g, char* buf, uint64_t cap, uint64_t* len);
+int atx_scene_add_node(atx_scene_graph* g, uint32_t parent, const char* name, const float* position,
+                       const float* rotation_xyzw, const float* scale, uint32_t* node);
+int atx_scene_set_node_transform(atx_scene_graph* g, uint32_t node, const float* position,
+                                 const float* rotation_xyzw, const float* scale);
+int atx_scene_add_sphere(atx_scene_graph* g, uint32_t node, const atx_sphere* s);
+int atx_scene_add_material(atx_scene_graph* g, const atx_material* m);
+int atx_scene_add_light(atx_scene_graph* g, const atx_light* l);
+int atx_scene_set_settings(atx_scene_graph* g, const atx_settings* s);
+int atx_scene_get_settings(const atx_scene_graph* g, atx_settings* s);
+int atx_scene_set_camera(atx_scene_graph* g, const float* position, const float* direction, float fov_deg);
+int atx_scene_get_camera(const atx_scene_graph* g, float* position, float* direction, float* fov_deg);
+int atx_scene_counts(const atx_scene_graph* g, uint64_t* nodes, uint64_t* spheres, uint64_t* materials,
+                     uint64_t* lights);
+/* Renderer::traverseSceneGraph (Renderer.cu:67-96): world-space spheres in
+ * pre-order (a node's spheres, then its children), centre = M*(c,1) / w,
+ * radius *= mean of the lengths of M's first three columns.  With out ==
+ * NULL only *count is set.                                                   */
+int atx_scene_flatten(const atx_scene_graph* g, atx_sphere* out, uint64_t cap, uint64_t* count);
+int atx_scene_get_materials(const atx_scene_graph* g, atx_material* out, uint64_t cap, uint64_t* count);
+int atx_scene_get_lights(const atx_scene_graph* g, atx_light* out, uint64_t cap, uint64_t* count);
+
+/* ---- Headless image output (replaces the Vulkan Image, Image.cpp:183-271) -- */
+/* Writes RGBA8 pixels ((a<<24)|(b<<16)|(g<<8)|r) as a binary PPM (P6, alpha
+ * dropped).  Row 0 of the buffer is the bottom of the view (clip y = -1);
+ * flip_vertical != 0 writes it last so that the file reads top-down.        */
+int atx_write_ppm(const char* path, const uint32_t* rgba, uint32_t width, uint32_t height, int flip_vertical);
+
 /* ---- Camera: host-side ray directions (Camera::UpdateRayDirection) -------- */
 /* Writes width*height*3 floats, bit-identical to what the kernels generate. */
 int atx_camera_ray_directions(const atx_camera* cam, float* dirs, uint64_t num_floats);
