@@ -172,6 +172,8 @@ PROTOTYPES = {
     "atx_scene_set_camera": (_I, [_P, _P, _P, C.c_float]),
     "atx_scene_get_camera": (_I, [_P, _P, _P, C.POINTER(C.c_float)]),
     "atx_scene_counts": (_I, [_P, C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64)]),
+    "atx_scene_get_node": (_I, [_P, _U32, C.c_char_p, _U64, _P, _P, _P, C.POINTER(_U32), C.POINTER(_U64)]),
+    "atx_scene_get_node_spheres": (_I, [_P, _U32, _P, _U64, C.POINTER(_U64)]),
     "atx_scene_flatten": (_I, [_P, _P, _U64, C.POINTER(_U64)]),
     "atx_scene_get_materials": (_I, [_P, _P, _U64, C.POINTER(_U64)]),
     "atx_scene_get_lights": (_I, [_P, _P, _U64, C.POINTER(_U64)]),

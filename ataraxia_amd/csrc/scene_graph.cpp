@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <new>
@@ -320,6 +321,7 @@ static Json jobj() { Json j; j.type = Json::Object; return j; }
 // ---------------------------------------------------------------------------
 struct Node {
     std::string name;
+    uint32_t parent = 0;
     float pos[3] = {0, 0, 0};
     float rot[4] = {0, 0, 0, 0};   // x, y, z, w; a value-initialised glm::quat is all zeros
     float scl[3] = {1, 1, 1};
@@ -483,6 +485,7 @@ static bool read_node(SceneGraph& g, const Json& j, uint32_t idx, std::string* e
             g.nodes.emplace_back();
             g.nodes[idx].children.push_back(k);
             g.nodes[k].name = "Untitled";
+            g.nodes[k].parent = idx;
             if (!read_node(g, c, k, err, depth + 1)) return false;
         }
     }
@@ -732,6 +735,7 @@ int atx_scene_add_node(atx_scene_graph* h, uint32_t parent, const char* name, co
     ATX_CHECK_ARG(parent < g->nodes.size(), "unknown parent node %u", parent);
     Node n;
     n.name = name ? name : "Untitled";
+    n.parent = parent;
     if (position) memcpy(n.pos, position, sizeof(n.pos));
     if (rotation_xyzw) memcpy(n.rot, rotation_xyzw, sizeof(n.rot));
     if (scale) memcpy(n.scl, scale, sizeof(n.scl));
@@ -825,6 +829,39 @@ int atx_scene_counts(const atx_scene_graph* h, uint64_t* nodes, uint64_t* sphere
     if (spheres) *spheres = ns;
     if (materials) *materials = g->materials.size();
     if (lights) *lights = g->lights.size();
+    return ATX_OK;
+}
+
+int atx_scene_get_node(const atx_scene_graph* h, uint32_t node, char* name, uint64_t name_cap, float* position,
+                       float* rotation_xyzw, float* scale, uint32_t* parent, uint64_t* num_spheres)
+{
+    const SceneGraph* g = cast(h);
+    ATX_CHECK_ARG(g && node < g->nodes.size(), "unknown node");
+    const Node& n = g->nodes[node];
+    if (name && name_cap > 0) {
+        const size_t k = std::min<size_t>(n.name.size(), (size_t)name_cap - 1);
+        memcpy(name, n.name.data(), k);
+        name[k] = '\0';
+    }
+    if (position) memcpy(position, n.pos, sizeof(n.pos));
+    if (rotation_xyzw) memcpy(rotation_xyzw, n.rot, sizeof(n.rot));
+    if (scale) memcpy(scale, n.scl, sizeof(n.scl));
+    if (parent) *parent = n.parent;
+    if (num_spheres) *num_spheres = n.spheres.size();
+    return ATX_OK;
+}
+
+int atx_scene_get_node_spheres(const atx_scene_graph* h, uint32_t node, atx_sphere* out, uint64_t cap,
+                               uint64_t* count)
+{
+    const SceneGraph* g = cast(h);
+    ATX_CHECK_ARG(g && count && node < g->nodes.size(), "bad argument");
+    const Node& n = g->nodes[node];
+    *count = n.spheres.size();
+    if (out) {
+        ATX_CHECK_ARG(cap >= n.spheres.size(), "buffer too small");
+        if (!n.spheres.empty()) memcpy(out, n.spheres.data(), n.spheres.size() * sizeof(atx_sphere));
+    }
     return ATX_OK;
 }
 

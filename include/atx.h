@@ -269,6 +269,13 @@ int atx_scene_set_camera(atx_scene_graph* g, const float* position, const float*
 int atx_scene_get_camera(const atx_scene_graph* g, float* position, float* direction, float* fov_deg);
 int atx_scene_counts(const atx_scene_graph* g, uint64_t* nodes, uint64_t* spheres, uint64_t* materials,
                      uint64_t* lights);
+/* Node `node` (ids are assigned parents-first, siblings in order): name
+ * (NUL-terminated, truncated to name_cap), transform, parent (0 for the root)
+ * and number of spheres.  Any output pointer may be NULL.                   */
+int atx_scene_get_node(const atx_scene_graph* g, uint32_t node, char* name, uint64_t name_cap, float* position,
+                       float* rotation_xyzw, float* scale, uint32_t* parent, uint64_t* num_spheres);
+int atx_scene_get_node_spheres(const atx_scene_graph* g, uint32_t node, atx_sphere* out, uint64_t cap,
+                               uint64_t* count);
 /* Renderer::traverseSceneGraph (Renderer.cu:67-96): world-space spheres in
  * pre-order (a node's spheres, then its children), centre = M*(c,1) / w,
  * radius *= mean of the lengths of M's first three columns.  With out ==
