@@ -35,6 +35,8 @@ ATX_ACCUM_NAIVE = 0
 ATX_ACCUM_KAHAN = 1
 ATX_KERNEL_PLAIN = 0
 ATX_KERNEL_WAVEFRONT = 1
+ATX_ACCEL_BVH = 0
+ATX_ACCEL_NONE = 1
 ATX_REQ_QUEUED, ATX_REQ_RUNNING, ATX_REQ_DONE, ATX_REQ_FAILED = 0, 1, 2, 3
 
 
@@ -76,7 +78,8 @@ class BatchTuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("frames_per_tick", C.c_uint32), ("chunk_px", C.c_uint32),
-                ("max_active", C.c_uint32), ("kahan", C.c_uint32)]
+                ("max_active", C.c_uint32), ("kahan", C.c_uint32), ("accel", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -94,7 +97,7 @@ class RequestInfo(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("sphere_tests", C.c_uint64),
-                ("exact_retraces", C.c_uint64), ("gpu_ms", C.c_double)]
+                ("exact_retraces", C.c_uint64), ("box_tests", C.c_uint64), ("gpu_ms", C.c_double)]
 
 
 assert C.sizeof(Sphere) == 20 and C.sizeof(Material) == 52 and C.sizeof(Light) == 28
@@ -135,10 +138,12 @@ PROTOTYPES = {
     "atx_renderer_get_stats": (_I, [_P, C.POINTER(RenderStats)]),
     "atx_renderer_render_debug": (_I, [_P, C.POINTER(CameraDesc), _P, _P, _P, _U32]),
     "atx_renderer_set_kernel": (_I, [_P, _I]),
+    "atx_renderer_set_accel": (_I, [_P, _I]),
     "atx_batch_create": (_I, [C.POINTER(_P)]),
     "atx_batch_destroy": (_I, [_P]),
     "atx_batch_add_scene": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(_U32)]),
     "atx_batch_set_accumulator": (_I, [_P, _I]),
+    "atx_batch_set_accel": (_I, [_P, _I]),
     "atx_batch_set_tuning": (_I, [_P, C.POINTER(BatchTuning)]),
     "atx_batch_get_tuning": (_I, [_P, C.POINTER(BatchTuning)]),
     "atx_batch_autotune": (_I, [_P, C.c_char_p, C.POINTER(BatchTuning)]),

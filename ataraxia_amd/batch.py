@@ -14,7 +14,7 @@ from .scene import FlatScene
 
 def _stats(s: _lib.RenderStats) -> dict:
     return {"samples": s.samples, "rays": s.rays, "sphere_tests": s.sphere_tests,
-            "exact_retraces": s.exact_retraces, "gpu_ms": s.gpu_ms}
+            "exact_retraces": s.exact_retraces, "box_tests": s.box_tests, "gpu_ms": s.gpu_ms}
 
 
 class Batch:
@@ -62,6 +62,10 @@ class Batch:
 
     def set_accumulator(self, kind: int) -> None:
         check(lib.atx_batch_set_accumulator(self._h, kind))
+
+    def set_accel(self, accel: int) -> None:
+        """ATX_ACCEL_BVH (default) or ATX_ACCEL_NONE (flat loop); same results."""
+        check(lib.atx_batch_set_accel(self._h, accel))
 
     def set_tuning(self, chunk_px: int = 0, workgroups: int = 0, use_graph: bool = True,
                    frames_per_launch: int = 0) -> None:

@@ -60,7 +60,8 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
     uint64_t pixels = 0, spheres = 0, spp = 0, bounces = 0, n = 0;
     batch_signature(b, &n, &pixels, &spheres, &spp, &bounces);
     std::ostringstream ks;
-    ks << device_key() << "|n" << n << "|px" << pixels << "|sph" << spheres << "|spp" << spp << "|b" << bounces;
+    ks << device_key() << "|n" << n << "|px" << pixels << "|sph" << spheres << "|spp" << spp << "|b" << bounces << "|acc"
+       << batch_accel(b) << "|k" << (batch_is_kahan(b) ? 1 : 0);
     const std::string key = ks.str();
 
     if (cache_path) {

@@ -39,6 +39,7 @@ struct Batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<std::unique_ptr<BatchEntry>> scenes;
     int accumulator = ATX_ACCUM_NAIVE;
+    int accel = ATX_ACCEL_BVH;
     atx_batch_tuning tuning{0, 0, 1, 0};
     DevBuf d_table, d_queue, counters;
     std::vector<uint32_t> order;          // table position -> scene id
@@ -115,7 +116,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             WfScene w;
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
                           b->kahan_lds ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan_lds,
-                          f0 + 1u, std::min(F, spp - f0), &w);
+                          f0 + 1u, std::min(F, spp - f0), b->accel, &w);
             w.chunk_begin = chunks;
             w.n_chunks = chunks_for(e.cam.width * e.cam.height, b->chunk_px);
             chunks += w.n_chunks;
@@ -187,6 +188,7 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
     b->stats.rays = ctr[kCtrRays];
     b->stats.sphere_tests = ctr[kCtrTests];
     b->stats.exact_retraces = ctr[kCtrExact];
+    b->stats.box_tests = ctr[kCtrBoxes];
     b->stats.gpu_ms = ms;
     if (ms_out) *ms_out = ms;
     return ATX_OK;
@@ -195,6 +197,8 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
 Batch* batch_cast(atx_batch* h) { return reinterpret_cast<Batch*>(h); }
 
 bool batch_is_kahan(const Batch* b) { return b->accumulator == ATX_ACCUM_KAHAN; }
+
+int batch_accel(const Batch* b) { return b->accel; }
 
 void batch_set_subset(Batch* b, const std::vector<uint32_t>& ids)
 {
@@ -302,6 +306,16 @@ int atx_batch_set_accumulator(atx_batch* h, int kind)
     ATX_CHECK_ARG(kind == ATX_ACCUM_NAIVE || kind == ATX_ACCUM_KAHAN, "unknown accumulator kind %d", kind);
     if (kind != b->accumulator) b->dirty = true;
     b->accumulator = kind;
+    return ATX_OK;
+}
+
+int atx_batch_set_accel(atx_batch* h, int accel)
+{
+    Batch* b = batch_cast(h);
+    ATX_CHECK_ARG(b, "batch is NULL");
+    ATX_CHECK_ARG(accel == ATX_ACCEL_BVH || accel == ATX_ACCEL_NONE, "unknown accel %d", accel);
+    if (accel != b->accel) b->dirty = true;
+    b->accel = accel;
     return ATX_OK;
 }
 

@@ -15,6 +15,8 @@
 #include <stdint.h>
 #include <hip/hip_runtime.h>
 
+#include "bvh.h"
+
 namespace atx {
 
 struct SceneDev {
@@ -26,6 +28,7 @@ struct SceneDev {
     uint32_t n_mat;
     uint32_t n_light;
     uint32_t pad_;
+    BvhDev bvh;              // n_nodes = 0: flat loop over all spheres
 };
 
 struct CameraDev {
@@ -40,7 +43,8 @@ struct CameraDev {
 enum CounterSlot : uint32_t {
     kCtrRays = 0,          // rays traced (continuation/primary + shadow)
     kCtrExact = 1,         // traces resolved by the exact fallback loop
-    kCtrTests = 2,         // ray-sphere tests (rays x spheres of their scene)
+    kCtrTests = 2,         // ray-sphere tests
+    kCtrBoxes = 3,         // ray-box tests of BVH nodes
     kCtrCount = 4,
 };
 
@@ -68,6 +72,7 @@ enum PlainVariant : uint32_t {
     kVarKahan = 1u << 1,     // compensated accumulation
     kVarDebug = 1u << 2,     // write per-pixel debug capture
     kVarExactLoop = 1u << 3, // closest-hit loop with exact sqrt/div per sphere
+    kVarBvh = 1u << 4,       // BVH traversal (sc.bvh); excludes kVarLds and kVarExactLoop
 };
 
 hipError_t launch_plain(const PlainArgs& a, uint32_t variant, hipStream_t stream);

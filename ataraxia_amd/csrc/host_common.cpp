@@ -66,7 +66,7 @@ int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material
 
 size_t scene_device_bytes(uint64_t n, uint64_t m, uint64_t l)
 {
-    return (16 * n + 16) + (4 * n + 16) + (48 * m + 16) + (32 * l + 16);
+    return (16 * n + 16) + (4 * n + 16) + (48 * m + 16) + (32 * l + 16) + bvh_device_bytes(n);
 }
 
 int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* materials, uint64_t m,
@@ -88,6 +88,35 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
     out->n_sph = (uint32_t)n;
     out->n_mat = (uint32_t)m;
     out->n_light = (uint32_t)l;
+
+    HostBvh T;
+    build_bvh(L.sph.data(), (uint32_t)n, &T);
+    out->bvh = BvhDev{};
+    if (T.n_nodes > 0) {
+        ATX_HIP(out->bvh_nodes.reserve(T.nodes.size() * 4 + 16));
+        ATX_HIP(out->bvh_lsph.reserve(T.lsph.size() * 4 + 16));
+        ATX_HIP(out->bvh_lidx.reserve(T.lidx.size() * 4 + 16));
+        ATX_HIP(out->bvh_big.reserve(T.big.size() * 4 + 16));
+        ATX_HIP(hipMemcpy(out->bvh_nodes.get(), T.nodes.data(), T.nodes.size() * 4, hipMemcpyHostToDevice));
+        ATX_HIP(hipMemcpy(out->bvh_lsph.get(), T.lsph.data(), T.lsph.size() * 4, hipMemcpyHostToDevice));
+        ATX_HIP(hipMemcpy(out->bvh_lidx.get(), T.lidx.data(), T.lidx.size() * 4, hipMemcpyHostToDevice));
+        if (!T.big.empty())
+            ATX_HIP(hipMemcpy(out->bvh_big.get(), T.big.data(), T.big.size() * 4, hipMemcpyHostToDevice));
+        BvhDev& b = out->bvh;
+        b.nodes = out->bvh_nodes.as<float4>();
+        b.lsph = out->bvh_lsph.as<float4>();
+        b.lidx = out->bvh_lidx.as<int32_t>();
+        b.big = out->bvh_big.as<int32_t>();
+        b.n_nodes = T.n_nodes;
+        b.n_big = (uint32_t)T.big.size();
+        b.cx = T.cx;
+        b.cy = T.cy;
+        b.cz = T.cz;
+        b.rb = T.rb;
+        b.r_min = T.r_min;
+        b.r_max = T.r_max;
+        b.cbox = T.cbox;
+    }
     return ATX_OK;
 }
 

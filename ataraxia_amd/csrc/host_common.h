@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/atx.h"
+#include "bvh.h"
 
 namespace atx {
 
@@ -93,18 +94,32 @@ struct HostSceneLayout {
 int build_scene_layout(const atx_sphere* spheres, uint64_t n, const atx_material* materials,
                        uint64_t m, const atx_light* lights, uint64_t l, HostSceneLayout* out);
 
-// One scene's geometry in HBM (device layout of device_types.h).
+// One scene's geometry in HBM (device layout of device_types.h), with its
+// sphere BVH (bvh.h; bvh.n_nodes = 0 when the scene has no tree).
 struct DevScene {
     DevBuf sph, sph_mat, mat, light;
+    DevBuf bvh_nodes, bvh_lsph, bvh_lidx, bvh_big;
     uint32_t n_sph = 0, n_mat = 0, n_light = 0;
-    size_t bytes() const { return sph.bytes() + sph_mat.bytes() + mat.bytes() + light.bytes(); }
+    BvhDev bvh{};
+    size_t bytes() const
+    {
+        return sph.bytes() + sph_mat.bytes() + mat.bytes() + light.bytes() + bvh_nodes.bytes() +
+               bvh_lsph.bytes() + bvh_lidx.bytes() + bvh_big.bytes();
+    }
+    // The tree to hand to a kernel: the scene's, or none for the flat loop.
+    BvhDev accel(int mode) const
+    {
+        BvhDev b{};
+        if (mode == ATX_ACCEL_BVH) b = bvh;
+        return b;
+    }
 };
 
 // Validates, converts and uploads (synchronously) a flattened scene.
 int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* materials, uint64_t m,
                  const atx_light* lights, uint64_t l, DevScene* out);
 
-// Device bytes a scene of this size occupies (geometry only).
+// Device bytes a scene of this size occupies (geometry and tree).
 size_t scene_device_bytes(uint64_t n, uint64_t m, uint64_t l);
 
 }  // namespace atx

@@ -12,6 +12,7 @@
 // Ray generation happens here too (from the inverse matrices the host
 // computes once per frame), so no per-pixel direction buffer crosses PCIe.
 #include "device_types.h"
+#include "trace_bvh.h"
 #include "trace_device.h"
 
 namespace atx {
@@ -36,6 +37,29 @@ __device__ __forceinline__ Hit trace(const Src& src, uint32_t n, f3 o, f3 d, uin
     return trace_filtered(src, n, o, d, exact_count);
 }
 
+struct TraceCount {
+    uint32_t exact = 0, tests = 0, boxes = 0;
+};
+
+// Closest hit (dist2 < 0) or shadow query (dist2 = squared light distance:
+// the returned sphere may be any occluder instead of the closest one).
+template <bool kLds, bool kExactLoop, bool kBvh>
+__device__ __forceinline__ Hit trace_ray(const PlainArgs& a, const LdsSrc& lsrc, const GlobalSrc& gsrc, f3 o, f3 d,
+                                         float dist2, TraceCount& c)
+{
+    if (kBvh) {
+        BvhQuery q;
+        const Hit h = trace_bvh(a.sc.bvh, gsrc, a.sc.n_sph, o, d, dist2, q);
+        c.exact += q.exact;
+        c.tests += q.tests;
+        c.boxes += q.boxes;
+        return h;
+    }
+    c.tests += a.sc.n_sph;
+    return kLds ? trace<LdsSrc, kExactLoop>(lsrc, a.sc.n_sph, o, d, c.exact)
+                : trace<GlobalSrc, kExactLoop>(gsrc, a.sc.n_sph, o, d, c.exact);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int off = 32; off > 0; off >>= 1)
@@ -43,7 +67,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-template <bool kLds, bool kKahan, bool kDebug, bool kExactLoop>
+template <bool kLds, bool kKahan, bool kDebug, bool kExactLoop, bool kBvh>
 __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
 {
     extern __shared__ float4 s_sph[];
@@ -56,7 +80,8 @@ __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
     const uint32_t x = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t W = a.cam.width, H = a.cam.height;
-    uint32_t rays = 0, exact = 0;
+    uint32_t rays = 0;
+    TraceCount cnt;
 
     if (x < W && y < H) {
         const size_t px = (size_t)x + (size_t)y * W;
@@ -77,8 +102,7 @@ __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
         const GlobalSrc gsrc{a.sc.sph};
         for (int i = 0; i < bounces; i++) {
             seed += (uint32_t)i;
-            const Hit ht = kLds ? trace<LdsSrc, kExactLoop>(lsrc, a.sc.n_sph, ro, rd, exact)
-                                : trace<GlobalSrc, kExactLoop>(gsrc, a.sc.n_sph, ro, rd, exact);
+            const Hit ht = trace_ray<kLds, kExactLoop, kBvh>(a, lsrc, gsrc, ro, rd, -1.0f, cnt);
             ++rays;
             if (kDebug && a.dbg_hit && i < (int)a.dbg_bounces)
                 a.dbg_hit[px * a.dbg_bounces + i] = ht.id;
@@ -112,8 +136,8 @@ __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
                 const float dist2 = dot(L, L);
                 L = normalize(L);
                 const f3 so = add(pos, scale(nrm, 0.0001f));
-                const Hit sh = kLds ? trace<LdsSrc, kExactLoop>(lsrc, a.sc.n_sph, so, L, exact)
-                                    : trace<GlobalSrc, kExactLoop>(gsrc, a.sc.n_sph, so, L, exact);
+                // the debug capture records the closest occluder; otherwise any proven one will do
+                const Hit sh = trace_ray<kLds, kExactLoop, kBvh>(a, lsrc, gsrc, so, L, kDebug ? -1.0f : dist2, cnt);
                 ++rays;
                 if (kDebug && a.dbg_shadow && i < (int)a.dbg_bounces)
                     a.dbg_shadow[px * a.dbg_bounces + i] = sh.id;
@@ -161,19 +185,23 @@ __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
     }
 
     const unsigned long long r = wave_sum(rays);
-    const unsigned long long e = wave_sum(exact);
+    const unsigned long long e = wave_sum(cnt.exact);
+    const unsigned long long ts = wave_sum(cnt.tests);
+    const unsigned long long bx = wave_sum(cnt.boxes);
     if (lane == 0) {
         if (r) atomicAdd(a.counters + kCtrRays, r);
         if (e) atomicAdd(a.counters + kCtrExact, e);
+        if (ts) atomicAdd(a.counters + kCtrTests, ts);
+        if (bx) atomicAdd(a.counters + kCtrBoxes, bx);
     }
 }
 
-template <bool kLds, bool kKahan, bool kDebug, bool kExactLoop>
+template <bool kLds, bool kKahan, bool kDebug, bool kExactLoop, bool kBvh = false>
 hipError_t launch_t(const PlainArgs& a, hipStream_t stream)
 {
     const dim3 grid((a.cam.width + 15u) / 16u, (a.cam.height + 15u) / 16u);
     const size_t lds = kLds ? (size_t)a.sc.n_sph * sizeof(float4) : 0;
-    hipLaunchKernelGGL((k_render_plain<kLds, kKahan, kDebug, kExactLoop>), grid, dim3(256), lds, stream, a);
+    hipLaunchKernelGGL((k_render_plain<kLds, kKahan, kDebug, kExactLoop, kBvh>), grid, dim3(256), lds, stream, a);
     return hipGetLastError();
 }
 
@@ -182,6 +210,13 @@ hipError_t launch_t(const PlainArgs& a, hipStream_t stream)
 hipError_t launch_plain(const PlainArgs& a, uint32_t v, hipStream_t stream)
 {
     const bool lds = v & kVarLds, kahan = v & kVarKahan, dbg = v & kVarDebug, ex = v & kVarExactLoop;
+    if (v & kVarBvh) {
+        if (a.sc.bvh.n_nodes == 0) return hipErrorInvalidValue;
+        if (!kahan && !dbg) return launch_t<false, false, false, false, true>(a, stream);
+        if (kahan && !dbg) return launch_t<false, true, false, false, true>(a, stream);
+        if (!kahan && dbg) return launch_t<false, false, true, false, true>(a, stream);
+        return launch_t<false, true, true, false, true>(a, stream);
+    }
 #define ATX_DISPATCH(L, K, D, E) \
     if (lds == L && kahan == K && dbg == D && ex == E) return launch_t<L, K, D, E>(a, stream);
     ATX_DISPATCH(true, false, false, false)

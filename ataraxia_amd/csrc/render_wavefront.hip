@@ -24,6 +24,7 @@
 // compiler lowers to scalar loads (two spheres per s_load_dwordx8, used
 // directly as VALU operands): no LDS staging, no barrier, waves independent.
 #include "device_types.h"
+#include "trace_bvh.h"
 #include "trace_device.h"
 #include "wavefront.h"
 
@@ -32,8 +33,27 @@ namespace {
 
 constexpr uint32_t kIdle = 0, kCont = 1, kShadow = 2, kWait = 3;
 
+// The scene table and the sphere arrays do not change during a launch; read
+// through the constant address space, wave-uniform loads from them become
+// scalar loads (s_load) into SGPRs even though the kernel stores to global
+// memory elsewhere.  (The host pass only parses the kernel; it gets no
+// address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ATX_CONST __attribute__((address_space(4)))
+#else
+#define ATX_CONST
+#endif
+using CScene = const ATX_CONST WfScene;
+using CF4 = const ATX_CONST float4;
+
+template <class T>
+__device__ __forceinline__ const ATX_CONST T* as_const(const T* p)
+{
+    return (const ATX_CONST T*)(uintptr_t)p;
+}
+
 struct SphSrc {
-    const float4* __restrict__ p;
+    CF4* p;
     __device__ __forceinline__ float4 operator()(uint32_t i) const { return p[i]; }
 };
 
@@ -75,8 +95,9 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
     char* wave_lds = lds + (size_t)wave * 2u * P * (20u + 16u * K);
 
     // ---- wave-uniform scheduling state ----
+    CScene* scenes = as_const(a.scenes);
     int32_t sid = -1;
-    const WfScene* S = nullptr;
+    CScene* S = nullptr;
     int32_t pend = -1;
     bool qdone = false;
     uint32_t cs = 0;                                   // slot whose samples are handed out
@@ -94,7 +115,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
     int32_t mid = 0;                // material of the current hit
     uint32_t lidx = 0;              // light of the current shadow ray
     uint32_t rays = 0, exact = 0;
-    unsigned long long tests = 0;
+    unsigned long long tests = 0, boxes = 0;
 
     for (;;) {
         // ---------------- write back chunks whose samples are all done ----------------
@@ -150,7 +171,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                 uint32_t got = 0;
                 if (lane == 0)
                     got = atomicAdd(a.queue, 1u);
-                got = __shfl(got, 0, 64);
+                got = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(got, 0, 64));
                 if (got >= a.total_chunks) {
                     qdone = true;
                     break;
@@ -158,8 +179,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                 c = (int32_t)got;
             }
             int32_t s = sid < 0 ? 0 : sid;
-            while (s + 1 < (int32_t)a.n_scenes &&
-                   (uint32_t)c >= a.scenes[s].chunk_begin + a.scenes[s].n_chunks)
+            while (s + 1 < (int32_t)a.n_scenes && (uint32_t)c >= scenes[s].chunk_begin + scenes[s].n_chunks)
                 ++s;
             if (s != sid) {
                 // a new scene: only once every lane and both slots are drained
@@ -168,7 +188,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                     break;
                 }
                 sid = s;
-                S = &a.scenes[s];
+                S = &scenes[s];
             }
             pend = -1;
             const uint32_t npx = S->width * S->height;
@@ -199,7 +219,12 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                 const uint32_t W = S->width;
                 const uint32_t y = px / W, x = px - y * W;
                 ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
-                rd = camera_ray(S->inv_proj, S->inv_view, x, y, W, S->height);
+                float ip[16], iv[16];
+                for (int k = 0; k < 16; ++k) {
+                    ip[k] = S->inv_proj[k];
+                    iv[k] = S->inv_view[k];
+                }
+                rd = camera_ray(ip, iv, x, y, W, S->height);
                 tp = mk(1.0f, 1.0f, 1.0f);
                 col = mk(0.0f, 0.0f, 0.0f);
                 seed = x + y * W;
@@ -225,9 +250,22 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
             const bool shadow = state == kShadow;
             const f3 o = shadow ? so : ro;
             const f3 d = shadow ? L : rd;
-            const Hit h = trace_filtered(SphSrc{S->sph}, S->n_sph, o, d, exact);
+            const SphSrc flat{as_const(S->sph)};
+            Hit h;
+            if (S->bvh.n_nodes) {
+                // the debug capture records the closest occluder; otherwise any proven one will do
+                const BvhDev B = S->bvh;
+                BvhQuery q;
+                const float qd2 = (shadow && S->dbg_shadow == nullptr) ? dist2 : -1.0f;
+                h = trace_bvh(B, flat, S->n_sph, o, d, qd2, q);
+                exact += q.exact;
+                tests += q.tests;
+                boxes += q.boxes;
+            } else {
+                h = trace_filtered(flat, S->n_sph, o, d, exact);
+                tests += S->n_sph;
+            }
             ++rays;
-            tests += S->n_sph;
 
             // ---------------- shading (divergent) ----------------
             if (!shadow) {
@@ -238,7 +276,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                         col = add(col, mul(mk(0.6f, 0.7f, 0.9f), tp));
                     end_path = true;
                 } else {
-                    const f3 center = xyz(S->sph[h.id]);
+                    const f3 center = xyz(as_const(S->sph)[h.id]);
                     f3 pos = add(sub(ro, center), scale(rd, h.t));
                     nrm = normalize(pos);
                     pos = add(pos, center);
@@ -325,10 +363,12 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
     const unsigned long long r = wave_sum(rays);
     const unsigned long long e = wave_sum(exact);
     const unsigned long long ts = wave_sum(tests);
+    const unsigned long long bx = wave_sum(boxes);
     if (lane == 0) {
         if (r) atomicAdd(a.counters + kCtrRays, r);
         if (e) atomicAdd(a.counters + kCtrExact, e);
         if (ts) atomicAdd(a.counters + kCtrTests, ts);
+        if (bx) atomicAdd(a.counters + kCtrBoxes, bx);
     }
 }
 

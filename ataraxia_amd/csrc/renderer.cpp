@@ -28,6 +28,7 @@ struct Renderer {
     DevScene geo;
     bool has_scene = false;
     int kernel = ATX_KERNEL_PLAIN;
+    int accel = ATX_ACCEL_BVH;
     DevBuf wf_table, wf_queue;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = kDefaultChunkPx;
@@ -67,6 +68,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     a.sc.n_sph = r->geo.n_sph;
     a.sc.n_mat = r->geo.n_mat;
     a.sc.n_light = r->geo.n_light;
+    const int accel = r->exact_loop ? ATX_ACCEL_NONE : r->accel;
+    a.sc.bvh = r->geo.accel(accel);
     camera_inverse_matrices(cam->position, cam->direction, cam->fov_deg, cam->near_clip, cam->far_clip,
                             cam->width, cam->height, a.cam.inv_proj, a.cam.inv_view);
     memcpy(a.cam.pos, cam->position, sizeof(a.cam.pos));
@@ -87,6 +90,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if ((size_t)r->geo.n_sph * sizeof(float4) <= r->lds_budget) variant |= kVarLds;
     if (r->accumulator == ATX_ACCUM_KAHAN) variant |= kVarKahan;
     if (r->exact_loop) variant |= kVarExactLoop;
+    if (a.sc.bvh.n_nodes) variant = (variant & ~(uint32_t)kVarLds) | kVarBvh;
     if (dbg_rad || dbg_hit || dbg_shadow) {
         variant |= kVarDebug;
         if (dbg_rad) { ATX_HIP(d_rad.reserve(npx * sizeof(float4))); a.dbg_radiance = d_rad.as<float4>(); }
@@ -106,7 +110,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         while (left > 0) {
             const uint32_t k = r->settings.accumulation ? (left < r->wf_frames ? left : r->wf_frames) : 1u;
             WfScene e;
-            fill_wf_scene(r->geo, r->settings, *cam, a.accum, a.comp, a.image, a.comp != nullptr, fi, k, &e);
+            fill_wf_scene(r->geo, r->settings, *cam, a.accum, a.comp, a.image, a.comp != nullptr, fi, k, accel,
+                          &e);
             e.chunk_begin = 0;
             e.n_chunks = chunks_for((uint32_t)npx, chunk);
             e.dbg_radiance = a.dbg_radiance;
@@ -165,8 +170,9 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     ATX_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
     r->stats.samples = (uint64_t)npx * frames;
     r->stats.rays = ctr[kCtrRays];
-    r->stats.sphere_tests = wf ? ctr[kCtrTests] : ctr[kCtrRays] * (uint64_t)r->geo.n_sph;
+    r->stats.sphere_tests = ctr[kCtrTests];
     r->stats.exact_retraces = ctr[kCtrExact];
+    r->stats.box_tests = ctr[kCtrBoxes];
     r->stats.gpu_ms = ms;
     return ATX_OK;
 }
@@ -260,6 +266,15 @@ int atx_renderer_set_kernel(atx_renderer* h, int kind)
     ATX_CHECK_ARG(r, "renderer is NULL");
     ATX_CHECK_ARG(kind == ATX_KERNEL_PLAIN || kind == ATX_KERNEL_WAVEFRONT, "unknown kernel kind %d", kind);
     r->kernel = kind;
+    return ATX_OK;
+}
+
+int atx_renderer_set_accel(atx_renderer* h, int accel)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(accel == ATX_ACCEL_BVH || accel == ATX_ACCEL_NONE, "unknown accel %d", accel);
+    r->accel = accel;
     return ATX_OK;
 }
 

@@ -175,7 +175,8 @@ static void worker_main(Server* s)
             WfScene w;
             fill_wf_scene(r->geo, r->settings, r->cam, r->accum.as<float4>(),
                           s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(),
-                          s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done), &w);
+                          s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done),
+                          (int)s->cfg.accel, &w);
             w.chunk_begin = chunks;
             w.n_chunks = chunks_for(r->cam.width * r->cam.height, P);
             chunks += w.n_chunks;
@@ -255,6 +256,8 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be 64..256, step 64");
     ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");
     s->cfg.kahan = s->cfg.kahan ? 1u : 0u;
+    ATX_CHECK_ARG(s->cfg.accel == ATX_ACCEL_BVH || s->cfg.accel == ATX_ACCEL_NONE, "unknown accel %u",
+                  s->cfg.accel);
     ATX_HIP(hipGetDevice(&s->device));
     if (!s->cfg.hbm_budget_bytes) {
         size_t free_b = 0, total_b = 0;

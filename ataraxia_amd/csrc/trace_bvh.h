@@ -1,0 +1,156 @@
+// trace_bvh.h -- closest-hit and shadow queries through the sphere BVH with
+// the flat loop's exact result (Engine/src/Renderer.cu:251-285 semantics).
+//
+// Which spheres a box may hide.  Let D bound |o - c| for the tree's spheres
+// (the ray origin o to a sphere centre c) and f(t) = |o + t d - c|^2 - r^2.
+// The reference's rounded evaluation (oc, b, c, disc, sqrt, numerator and
+// quotient each rounded once, u = 2^-24) returns a t~ with
+//   |f(t~)| <= 93 u D^2 + 36 u r^2 < 2^-17 (D^2 + r^2)
+// (forward error of b within 4.01 u D, of c within 6.02 u D^2 + u r^2; the
+// disc rounding moves c by at most 2.02 u (2 D^2 + r^2); the root of the
+// perturbed quadratic then satisfies |f| <= 48.2 u D^2 + 16.1 u r^2, and the
+// last three roundings move t by at most u (5.03 D + 3.02 r) on a slope of at
+// most 6 D + 2 r).  So every hit the flat loop can report lies in the sphere
+// of radius sqrt(r^2 + 2^-17 (D^2 + r^2)): the box inflation per ray is the
+// largest excess of that radius over r across the tree's radius range
+// (attained at r_min or r_max), plus 2^-19 (|o|_inf + cbox + m) for the
+// rounding of the slab test itself (at most ~5.4 u of those magnitudes).
+//
+// Culling by distance uses an upper bound of the best sphere's exact t
+// (best_hi / a, rounded up by 2^-20): a box whose inflated entry lies beyond
+// it holds only spheres with a strictly larger exact t, which the reference
+// rejects whatever their index.  Shadow queries (dist2 >= 0: only the
+// reference's decision t > 0 && t*t < dist2 on the closest hit matters) also
+// skip boxes beyond sqrt(dist2), where no sphere can satisfy t*t < dist2, and
+// stop at the first sphere whose exact t is proven below sqrt(dist2); the
+// sphere they return is then some occluder, not necessarily the closest.
+//
+// Spheres that are visited go through filter_sphere, like the flat loop; a
+// lane whose filter is ambiguous re-runs the exact flat loop, and origins so
+// far away that the inflation cannot be formed use the flat filtered loop.
+#pragma once
+
+#include "bvh.h"
+#include "trace_device.h"
+
+namespace atx {
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ATX_GLOBAL __attribute__((address_space(1)))
+#else
+#define ATX_GLOBAL
+#endif
+
+struct BvhQuery {
+    uint32_t tests = 0;   // sphere tests
+    uint32_t boxes = 0;   // node box tests
+    uint32_t exact = 0;   // exact fallback loops
+};
+
+__device__ __forceinline__ float bvh_inv_dir(float v)
+{
+    const float tiny = 0x1p-64f;
+    const float c = __builtin_fabsf(v) < tiny ? __builtin_copysignf(tiny, v) : v;
+    return __builtin_amdgcn_rcpf(c);
+}
+
+// Radius excess sqrt(r^2 + q) - r, q = 2^-17 (D^2 + r^2), in a form without
+// cancellation.
+__device__ __forceinline__ float bvh_excess(float r, float D2)
+{
+    const float q = 0x1p-17f * (D2 + r * r);
+    return q / (__builtin_amdgcn_sqrtf(r * r + q) + r);
+}
+
+// Closest hit (dist2 < 0) or shadow query (dist2 >= 0) of ray (o, d).
+// `flat` reads sphere i of the scene in its original order (n_all spheres).
+template <class FlatSrc>
+__device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, uint32_t n_all, f3 o, f3 d,
+                                         float dist2, BvhQuery& q)
+{
+    const float a = dot(d, d);
+    const bool shadow = dist2 >= 0.0f;
+    Filter f = filter_init();
+
+    // spheres outside the tree (wave-uniform indices)
+    for (uint32_t j = 0; j < B.n_big; ++j) {
+        const int32_t id = B.big[j];
+        filter_sphere(flat((uint32_t)id), id, o, d, a, f);
+    }
+    q.tests += B.n_big;
+
+    // per-ray box inflation
+    const float ox = o.x - B.cx, oy = o.y - B.cy, oz = o.z - B.cz;
+    const float dist = __builtin_amdgcn_sqrtf((ox * ox + oy * oy) + oz * oz);
+    const float D = (dist * (1.0f + 0x1p-18f) + B.rb) * (1.0f + 0x1p-20f);
+    const float D2 = D * D;
+    const float m = __builtin_fmaxf(bvh_excess(B.r_min, D2), bvh_excess(B.r_max, D2)) * (1.0f + 0x1p-16f);
+    const float omax = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(o.x), __builtin_fabsf(o.y)), __builtin_fabsf(o.z));
+    const float e = 0x1p-19f * ((omax + B.cbox) + m) + 0x1p-60f;
+    const float M = (m + e) * (1.0f + 0x1p-20f);
+    if (!(M < 0x1p60f) || !(omax < 0x1p60f)) {
+        // origin too far for the inflation to be formed: flat loop over everything
+        q.tests += n_all;
+        uint32_t ex = 0;
+        const Hit h = trace_filtered(flat, n_all, o, d, ex);
+        q.exact += ex;
+        return h;
+    }
+    const float ix = bvh_inv_dir(d.x), iy = bvh_inv_dir(d.y), iz = bvh_inv_dir(d.z);
+    const float px = -((o.x + M) * ix), py = -((o.y + M) * iy), pz = -((o.z + M) * iz);
+    const float qx = -((o.x - M) * ix), qy = -((o.y - M) * iy), qz = -((o.z - M) * iz);
+    const float inv_a_up = __builtin_amdgcn_rcpf(a) * (1.0f + 0x1p-20f);
+    const float s_up = shadow ? __builtin_amdgcn_sqrtf(dist2) * (1.0f + 0x1p-18f) : __builtin_inff();
+    float tlim = __builtin_fminf(f.best_hi * inv_a_up, s_up);
+
+    const uint32_t n_nodes = B.n_nodes;
+    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    const ATX_GLOBAL float4* nodes = (const ATX_GLOBAL float4*)(uintptr_t)B.nodes;
+    const ATX_GLOBAL float4* lsph = (const ATX_GLOBAL float4*)(uintptr_t)B.lsph;
+    const ATX_GLOBAL int32_t* lidx = (const ATX_GLOBAL int32_t*)(uintptr_t)B.lidx;
+    const uint32_t base = oct * n_nodes * 2u;
+    uint32_t i = (shadow && tlim * tlim < dist2) ? n_nodes : 0u;
+    while (i < n_nodes) {
+        const float4 n0 = nodes[base + 2u * i];
+        const float4 n1 = nodes[base + 2u * i + 1u];
+        ++q.boxes;
+        const float ax = __builtin_fmaf(n0.x, ix, px), bx = __builtin_fmaf(n1.x, ix, qx);
+        const float ay = __builtin_fmaf(n0.y, iy, py), by = __builtin_fmaf(n1.y, iy, qy);
+        const float az = __builtin_fmaf(n0.z, iz, pz), bz = __builtin_fmaf(n1.z, iz, qz);
+        const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(ax, bx), __builtin_fminf(ay, by)),
+                                         __builtin_fmaxf(__builtin_fminf(az, bz), 0.0f));
+        const float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
+                                         __builtin_fminf(__builtin_fmaxf(az, bz), tlim));
+        uint32_t next = __float_as_uint(n0.w);
+        if (tn <= tf) {
+            const uint32_t leaf = __float_as_uint(n1.w);
+            if (leaf == 0u) {
+                next = i + 1u;
+            } else {
+                const uint32_t first = leaf >> 3, cnt = leaf & 7u;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    if (filter_sphere(lsph[first + k], lidx[first + k], o, d, a, f))
+                        tlim = __builtin_fminf(tlim, f.best_hi * inv_a_up);
+                }
+                q.tests += cnt;
+                // t of the best sphere <= tlim, and tlim*tlim < dist2 proves it occludes
+                if (shadow && tlim * tlim < dist2)
+                    next = n_nodes;
+            }
+        }
+        i = next;
+    }
+
+    Hit h;
+    if (f.amb) {
+        ++q.exact;
+        q.tests += n_all;
+        h = trace_exact(flat, n_all, o, d);
+    } else {
+        h.id = f.best;
+        h.t = f.best >= 0 ? exact_t(flat((uint32_t)f.best), o, d, a) : FLT_MAX;
+    }
+    return h;
+}
+
+}  // namespace atx

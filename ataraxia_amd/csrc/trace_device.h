@@ -79,36 +79,61 @@ __device__ __forceinline__ float exact_t(float4 s, f3 o, f3 d, float a)
     return (-b - sqrt_cr(disc)) / a;
 }
 
+// Running state of the filtered closest-hit search of one ray.
+struct Filter {
+    float best_lo;   // interval of the best numerator so far
+    float best_hi;
+    int32_t best;    // its sphere index, -1 = none
+    bool amb;        // the filter could not decide: re-run the exact loop
+};
+
+__device__ __forceinline__ Filter filter_init()
+{
+    Filter f;
+    f.best_lo = __builtin_inff();
+    f.best_hi = __builtin_inff();
+    f.best = -1;
+    f.amb = false;
+    return f;
+}
+
+// One filtered test of sphere s (index id) against the ray (o, d), a = d.d.
+// The decision does not depend on the order in which spheres are visited:
+// "better" means a strictly smaller exact t, "near" an interval that
+// overlaps the best one.  Returns true when s became the new best.
+__device__ __forceinline__ bool filter_sphere(float4 s, int32_t id, f3 o, f3 d, float a, Filter& f)
+{
+    constexpr float kMargin = 0x1p-17f;
+    constexpr float kTinyDisc = 0x1p-96f;
+    const float ocx = o.x - s.x, ocy = o.y - s.y, ocz = o.z - s.z;
+    const float b = (ocx * d.x + ocy * d.y) + ocz * d.z;
+    const float c = ((ocx * ocx + ocy * ocy) + ocz * ocz) - s.w;
+    const float disc = b * b - a * c;
+    const float sq = __builtin_amdgcn_sqrtf(disc);  // NaN when disc < 0
+    const float nn = -b - sq;
+    const float u = sq + __builtin_fabsf(nn);
+    const float hi = __builtin_fmaf(u, kMargin, nn);
+    const float lo = __builtin_fmaf(u, -kMargin, nn);
+    const bool better = (hi < f.best_lo) & (lo > 0.0f);
+    const bool near = (hi > 0.0f) & (lo < f.best_hi);
+    const bool tiny = (disc > 0.0f) & (disc < kTinyDisc);
+    f.amb |= (near & !better) | tiny;
+    f.best_lo = better ? lo : f.best_lo;
+    f.best_hi = better ? hi : f.best_hi;
+    f.best = better ? id : f.best;
+    return better;
+}
+
 template <class Src>
 __device__ __forceinline__ Hit trace_filtered(const Src& src, uint32_t n, f3 o, f3 d, uint32_t& exact_count)
 {
     const float a = dot(d, d);
-    constexpr float kMargin = 0x1p-17f;
-    constexpr float kTinyDisc = 0x1p-96f;
-    float best_lo = __builtin_inff();
-    float best_hi = __builtin_inff();
-    int32_t best = -1;
-    bool amb = false;
+    Filter f = filter_init();
 #pragma unroll 2
-    for (uint32_t i = 0; i < n; ++i) {
-        const float4 s = src(i);
-        const float ocx = o.x - s.x, ocy = o.y - s.y, ocz = o.z - s.z;
-        const float b = (ocx * d.x + ocy * d.y) + ocz * d.z;
-        const float c = ((ocx * ocx + ocy * ocy) + ocz * ocz) - s.w;
-        const float disc = b * b - a * c;
-        const float sq = __builtin_amdgcn_sqrtf(disc);  // NaN when disc < 0
-        const float nn = -b - sq;
-        const float u = sq + __builtin_fabsf(nn);
-        const float hi = __builtin_fmaf(u, kMargin, nn);
-        const float lo = __builtin_fmaf(u, -kMargin, nn);
-        const bool better = (hi < best_lo) & (lo > 0.0f);
-        const bool near = (hi > 0.0f) & (lo < best_hi);
-        const bool tiny = (disc > 0.0f) & (disc < kTinyDisc);
-        amb |= (near & !better) | tiny;
-        best_lo = better ? lo : best_lo;
-        best_hi = better ? hi : best_hi;
-        best = better ? (int32_t)i : best;
-    }
+    for (uint32_t i = 0; i < n; ++i)
+        filter_sphere(src(i), (int32_t)i, o, d, a, f);
+    const bool amb = f.amb;
+    const int32_t best = f.best;
     Hit h;
     if (amb) {
         ++exact_count;

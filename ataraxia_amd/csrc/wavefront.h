@@ -21,6 +21,8 @@
 #include <stdint.h>
 #include <hip/hip_runtime.h>
 
+#include "bvh.h"
+
 namespace atx {
 
 struct WfScene {
@@ -52,6 +54,7 @@ struct WfScene {
     float4* dbg_radiance;  // optional per-pixel capture of the last frame (nullptr = off)
     int32_t* dbg_hit;
     int32_t* dbg_shadow;
+    BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
 struct WfPass {

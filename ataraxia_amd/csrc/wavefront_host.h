@@ -13,7 +13,7 @@ namespace atx {
 
 inline void fill_wf_scene(const DevScene& g, const atx_settings& st, const atx_camera& cam, float4* accum,
                           float4* comp, uint32_t* image, bool kahan, uint32_t frame0, uint32_t frames,
-                          WfScene* out)
+                          int accel, WfScene* out)
 {
     memset(out, 0, sizeof(*out));
     out->sph = g.sph.as<float4>();
@@ -38,6 +38,7 @@ inline void fill_wf_scene(const DevScene& g, const atx_settings& st, const atx_c
     out->accum = accum;
     out->comp = comp;
     out->image = image;
+    out->bvh = g.accel(accel);
 }
 
 inline uint32_t chunks_for(uint32_t pixels, uint32_t chunk_items)

@@ -1,0 +1,34 @@
+#!/bin/bash
+# rocprofv3 passes behind profiles/plain_1080p_k_render_<kernel>.txt.  Run from
+# the repo root on a machine with the GPU:
+#     KERNEL=plain|wavefront bash ataraxia_amd/profiling/profile_1080p.sh
+#   1. kernel trace + stats of bench.py (STEPS frames of the 1080p workload)
+#   2. FETCH_SIZE, then WRITE_SIZE, each in its own pass, over one launch:
+#      PMC_STEPS frames (plain: one launch per frame; wavefront: 16 frames
+#      per launch, so PMC_STEPS=16 is exactly one launch)
+#   3. the same two counters on copy_probe (256 MiB copy) for calibration
+#   4. SQ counters (waves, VALU instructions, active VALU cycles, LDS)
+# Every pass has its own time limit; the script stops at the first failure.
+set -u
+REPO=$(pwd)
+KERNEL=${KERNEL:-wavefront}
+OUT=${OUT:-$REPO/gpurun_out/prof_$KERNEL}
+STEPS=${STEPS:-32}
+if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-16}; fi
+BENCH=("$REPO/bench.py" --kernel "$KERNEL" --no-serve --no-cpu-baseline)
+mkdir -p "$OUT"
+export TMPDIR=/tmp
+run() { echo "== $*"; "$@"; local rc=$?; echo "== rc=$rc"; return $rc; }
+cd /tmp || exit 1
+run timeout -k 10 240 rocprofv3 --kernel-trace --stats -d "$OUT/kt" -o k -- \
+    python3 "${BENCH[@]}" --steps "$STEPS" --warmup 0 &&
+run timeout -s KILL 90 rocprofv3 --pmc FETCH_SIZE -d "$OUT/fetch" -o k -- \
+    python3 "${BENCH[@]}" --steps "$PMC_STEPS" --warmup 0 &&
+run timeout -s KILL 90 rocprofv3 --pmc WRITE_SIZE -d "$OUT/write" -o k -- \
+    python3 "${BENCH[@]}" --steps "$PMC_STEPS" --warmup 0 &&
+run timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE -d "$OUT/copy_fetch" -o copy -- \
+    "$REPO/ataraxia_amd/profiling/copy_probe" 268435456 &&
+run timeout -s KILL 60 rocprofv3 --pmc WRITE_SIZE -d "$OUT/copy_write" -o copy -- \
+    "$REPO/ataraxia_amd/profiling/copy_probe" 268435456 &&
+run timeout -s KILL 90 rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES -d "$OUT/sq" -o k -- \
+    python3 "${BENCH[@]}" --steps "$PMC_STEPS" --warmup 0

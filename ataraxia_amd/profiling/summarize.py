@@ -2,7 +2,7 @@
 
 usage: python ataraxia_amd/profiling/summarize.py <prof_dir> [--copy-bytes N]
 
-<prof_dir> is the OUT directory of profile_plain.sh (or a script of the same
+<prof_dir> is the OUT directory of profile_1080p.sh (or a script of the same
 shape): kt/ (kernel trace + stats), fetch/ and write/ (FETCH_SIZE and
 WRITE_SIZE passes), copy_fetch/ and copy_write/ (the same counters on
 copy_probe), sq/ (SQ counters).  Prints kernel durations, per-dispatch

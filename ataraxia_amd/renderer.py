@@ -86,6 +86,10 @@ class Renderer:
         """ATX_KERNEL_PLAIN (one lane per pixel) or ATX_KERNEL_WAVEFRONT."""
         check(lib.atx_renderer_set_kernel(self._h, kind))
 
+    def set_accel(self, accel: int) -> None:
+        """ATX_ACCEL_BVH (default) or ATX_ACCEL_NONE (flat loop); same results."""
+        check(lib.atx_renderer_set_accel(self._h, accel))
+
     # -- Renderer::Render ----------------------------------------------------------
     def render(self, camera: Camera, scene: FlatScene) -> None:
         if self._scene_ref is not scene or self.frame_index == 1:
@@ -119,7 +123,7 @@ class Renderer:
         s = _lib.RenderStats()
         check(lib.atx_renderer_get_stats(self._h, C.byref(s)))
         return {"samples": s.samples, "rays": s.rays, "sphere_tests": s.sphere_tests,
-                "exact_retraces": s.exact_retraces, "gpu_ms": s.gpu_ms}
+                "exact_retraces": s.exact_retraces, "box_tests": s.box_tests, "gpu_ms": s.gpu_ms}
 
 
 def camera_ray_directions(camera: Camera) -> np.ndarray:

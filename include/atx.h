@@ -14,6 +14,7 @@
  *   atx_renderer_get_image               Renderer::getImage + Image pixels   Engine/include/Renderer.h:23, Renderer.cu:240-242
  *   atx_renderer_get/set_settings        Renderer::getSettings/setSettings   Engine/include/Renderer.h:25-26
  *   atx_renderer_reset_frame_index       Renderer::resetFrameIndex           Engine/include/Renderer.h:28
+ *   atx_renderer_set_accel               Renderer::traceRay sphere loop      Engine/src/Renderer.cu:251-285
  *   atx_camera_ray_directions            Camera::UpdateRayDirection          Engine/src/Camera.cpp:161-195
  *   atx_scene_* (scene.json)             Utils::importScene / exportScene    Engine/src/Utils.cpp:53-62,175-187
  *
@@ -79,12 +80,21 @@ typedef struct atx_camera {
 #define ATX_ACCUM_NAIVE 0
 #define ATX_ACCUM_KAHAN 1
 
+/* Which spheres a ray tests (Renderer.cu:251-285 tests all of them).  BVH
+ * (the default) walks a bounding-volume hierarchy built at scene upload and
+ * gives the same closest hit, bit for bit, including the lowest index among
+ * equal distances; NONE is the flat loop over every sphere.  Scenes with
+ * fewer than 16 spheres always use the flat loop.                           */
+#define ATX_ACCEL_BVH 0
+#define ATX_ACCEL_NONE 1
+
 /* Counters of the last render call. */
 typedef struct atx_render_stats {
     uint64_t samples;        /* path samples (pixels x frames)                 */
     uint64_t rays;           /* rays traced: primary/continuation + shadow      */
-    uint64_t sphere_tests;   /* rays x spheres: ray-sphere tests performed      */
+    uint64_t sphere_tests;   /* ray-sphere tests performed                      */
     uint64_t exact_retraces; /* traces resolved by the exact fallback loop      */
+    uint64_t box_tests;      /* ray-box tests of BVH nodes                      */
     double gpu_ms;           /* HIP-event time of the render kernels            */
 } atx_render_stats;
 
@@ -136,6 +146,8 @@ int atx_renderer_render_debug(atx_renderer* r, const atx_camera* cam, float* rad
 #define ATX_KERNEL_PLAIN 0
 #define ATX_KERNEL_WAVEFRONT 1
 int atx_renderer_set_kernel(atx_renderer* r, int kind);
+/* ATX_ACCEL_BVH (default) or ATX_ACCEL_NONE; results are identical.        */
+int atx_renderer_set_accel(atx_renderer* r, int accel);
 
 /* ---- Batch: many scenes, persistent launches of several frames each ------ */
 /* No reference counterpart: the reference renders one scene per Render call.
@@ -167,6 +179,7 @@ int atx_batch_create(atx_batch** out);
 int atx_batch_destroy(atx_batch* b);
 int atx_batch_add_scene(atx_batch* b, const atx_scene_desc* desc, uint32_t* scene_id);
 int atx_batch_set_accumulator(atx_batch* b, int kind);
+int atx_batch_set_accel(atx_batch* b, int accel);
 int atx_batch_set_tuning(atx_batch* b, const atx_batch_tuning* t);
 int atx_batch_get_tuning(const atx_batch* b, atx_batch_tuning* t);
 /* Times candidate tunings on this batch (its first frames only) and applies
@@ -205,6 +218,8 @@ typedef struct atx_server_config {
     uint32_t chunk_px;          /* pixels per work chunk (0 = 128)                          */
     uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
     uint32_t kahan;             /* 1 = compensated accumulation                             */
+    uint32_t accel;             /* ATX_ACCEL_BVH (0) or ATX_ACCEL_NONE                      */
+    uint32_t reserved;
 } atx_server_config;
 
 typedef struct atx_server_stats {
