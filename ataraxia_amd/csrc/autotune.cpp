@@ -1,12 +1,12 @@
 // autotune.cpp -- picks the batch renderer's chunk size and persistent grid by
 // timing candidates on the batch itself, and caches the choice.
 //
-// Candidates: chunk_px in {64, 128, 256} x frames per launch in {16, 32} x
-// grid in {full, 3/4} of the device's resident-workgroup capacity.  Each
-// candidate renders the first F frames of a spread-out subsample of at most 8
-// of the batch's scenes twice (the second run is timed with HIP events; the
-// graph is re-captured per candidate) and the one with the highest samples/ms
-// wins.  The cache is a text file of
+// Candidates: chunk_px in {64, 128, 256} x frames per launch in {16, 32, 64}
+// on the full resident-workgroup grid, then the winner on 3/4 of the grid.
+// Each candidate renders the first F frames of a spread-out subsample of the
+// batch's scenes (up to 16 Mpixel, at least 8 scenes) twice (the second run is
+// timed with HIP events; the graph is re-captured per candidate) and the one
+// with the highest samples/ms wins.  The cache is a text file of
 // "<key>\t<chunk_px>\t<workgroups>\t<frames_per_launch>" lines; the key
 // names the device (arch, CU count) and the workload (scene count, pixels,
 // spheres, spp, bounces).
@@ -81,42 +81,61 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
         }
     }
 
-    // candidates: chunk size x frames per launch x grid fill, each timed on a
-    // spread-out subsample of at most 8 scenes (their first F frames, run twice,
-    // the second run timed)
+    // Candidates: chunk size x frames per launch on the full grid, then the
+    // winner on 3/4 of it.  Each is timed on a spread-out subsample of the
+    // scenes (every k-th scene until kTunePixels pixels, at least 8 scenes):
+    // their first F frames, run twice, the second run timed.  A launch's tail
+    // and scene switches weigh in the timing as they do in the full batch.
+    constexpr uint64_t kTunePixels = 16ull << 20;
     std::vector<uint32_t> sample;
     const size_t total = batch_size(b);
-    const size_t step = total > 8 ? total / 8 : 1;
-    for (size_t i = 0; i < total && sample.size() < 8; i += step) sample.push_back((uint32_t)i);
+    for (size_t stride = 1; stride <= total; stride *= 2) {
+        std::vector<uint32_t> pick;
+        uint64_t px = 0;
+        for (size_t i = 0; i < total; i += stride) {
+            pick.push_back((uint32_t)i);
+            px += batch_scene_pixels(b, (uint32_t)i);
+        }
+        sample = pick;
+        if (px <= kTunePixels || pick.size() <= 8) break;
+    }
     batch_set_subset(b, sample);
-    const uint32_t chunks[] = {64u, 128u, 256u};
-    const uint32_t frames[] = {16u, 32u};
     double best_rate = -1.0;
     atx_batch_tuning best{kDefaultChunkPx, 0, 1, kDefaultFramesPerLaunch};
+    auto try_one = [&](const atx_batch_tuning& t) -> int {
+        atx_batch_tuning tt = t;
+        int e = atx_batch_set_tuning(h, &tt);
+        if (e != ATX_OK) return e;
+        double ms = 0.0;
+        e = batch_run(b, t.frames_per_launch, &ms);   // warm-up (also captures the graph)
+        if (e == ATX_OK) e = batch_run(b, t.frames_per_launch, &ms);
+        if (e != ATX_OK) return e;
+        atx_render_stats st{};
+        atx_batch_get_stats(h, &st);
+        const double rate = ms > 0.0 ? (double)st.samples / ms : 0.0;   // samples per ms
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = t;
+        }
+        return ATX_OK;
+    };
+    const uint32_t chunks[] = {64u, 128u, 256u};
+    const uint32_t frames[] = {16u, 32u, 64u};
     for (uint32_t c : chunks) {
-        const uint32_t full = wavefront_full_grid(c, cur_kahan(h));
-        const uint32_t grids[] = {full, (full * 3) / 4};
         for (uint32_t F : frames) {
-            for (uint32_t g : grids) {
-                if (g == 0) continue;
-                atx_batch_tuning t{c, g, 1, F};
-                rc = atx_batch_set_tuning(h, &t);
-                if (rc != ATX_OK) return rc;
-                double ms = 0.0;
-                rc = batch_run(b, F, &ms);   // warm-up (also captures the graph)
-                if (rc == ATX_OK) rc = batch_run(b, F, &ms);
-                if (rc != ATX_OK) {
-                    batch_set_subset(b, {});
-                    return rc;
-                }
-                atx_render_stats st{};
-                atx_batch_get_stats(h, &st);
-                const double rate = ms > 0.0 ? (double)st.samples / ms : 0.0;   // samples per ms
-                if (rate > best_rate) {
-                    best_rate = rate;
-                    best = t;
-                }
+            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(c, cur_kahan(h)), 1, F});
+            if (rc != ATX_OK) {
+                batch_set_subset(b, {});
+                return rc;
             }
+        }
+    }
+    const uint32_t three_quarters = (best.workgroups * 3) / 4;
+    if (three_quarters > 0) {
+        rc = try_one(atx_batch_tuning{best.chunk_px, three_quarters, 1, best.frames_per_launch});
+        if (rc != ATX_OK) {
+            batch_set_subset(b, {});
+            return rc;
         }
     }
     batch_set_subset(b, {});

@@ -208,6 +208,12 @@ void batch_set_subset(Batch* b, const std::vector<uint32_t>& ids)
 
 size_t batch_size(const Batch* b) { return b->scenes.size(); }
 
+uint64_t batch_scene_pixels(const Batch* b, uint32_t id)
+{
+    const BatchEntry& e = *b->scenes[id];
+    return (uint64_t)e.cam.width * e.cam.height;
+}
+
 void batch_signature(const Batch* b, uint64_t* scenes, uint64_t* pixels, uint64_t* spheres, uint64_t* spp,
                      uint64_t* bounces)
 {

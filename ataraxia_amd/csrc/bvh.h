@@ -10,14 +10,15 @@
 // trace_bvh.h for the bound that sizes the per-ray box inflation.
 //
 // Layout in HBM (per scene):
-//   nodes  float4[kBvhLayouts][n_nodes][2]
+//   nodes  float4[layouts][n_nodes][2]   (layouts = 1 or 8)
 //          [0] = (bmin.x, bmin.y, bmin.z, skip)   skip: node index after this
 //                                                 node's subtree (n_nodes = end)
 //          [1] = (bmax.x, bmax.y, bmax.z, leaf)   leaf: 0 for an inner node,
 //                                                 else (first << 3) | count
-//          Each layout is the same tree in depth-first order; layout k visits,
-//          at every inner node, first the child nearer along the split axis
-//          for ray directions whose sign bits are k (x: 1, y: 2, z: 4).  The
+//          Each layout is the same tree in depth-first order; with 8 layouts,
+//          layout k visits, at every inner node, first the child nearer along
+//          the split axis for ray directions whose sign bits are k (x: 1,
+//          y: 2, z: 4); a single layout always visits the lower child first.  The
 //          traversal is stackless: a hit inner node continues at index + 1
 //          (its first child), anything else at `skip`.
 //   lsph   float4[n_tree]   tree spheres in leaf order (cx, cy, cz, r*r)
@@ -43,11 +44,11 @@ struct BvhDev {
     const int32_t* big;
     uint32_t n_nodes;   // nodes per layout; 0 = no tree (flat loop over all spheres)
     uint32_t n_big;
+    uint32_t layout_mask;  // 7: one layout per direction octant, 0: one layout
     float cx, cy, cz;   // centre of the tree's region
     float rb;           // upper bound of |centre - sphere centre| over tree spheres
     float r_min, r_max; // lower / upper bound of the tree spheres' radii
     float cbox;         // upper bound of |coordinate| over all node boxes
-    uint32_t pad_;
 };
 
 }  // namespace atx

@@ -173,9 +173,10 @@ bool finite_sphere(const float* s)
 
 }  // namespace
 
-void build_bvh(const float* sph4, uint32_t n, HostBvh* out)
+void build_bvh(const float* sph4, uint32_t n, uint32_t layouts, HostBvh* out)
 {
     *out = HostBvh();
+    layouts = layouts >= kBvhLayouts ? kBvhLayouts : 1u;
     if (n < kBvhMinSpheres) return;
 
     std::vector<int32_t> big;
@@ -255,8 +256,9 @@ void build_bvh(const float* sph4, uint32_t n, HostBvh* out)
     B.build(0, (int)P.size());
     const uint32_t nn = (uint32_t)B.nodes.size();
     out->n_nodes = nn;
-    out->nodes.assign((size_t)kBvhLayouts * nn * 8, 0.0f);
-    for (uint32_t oct = 0; oct < kBvhLayouts; ++oct) {
+    out->layouts = layouts;
+    out->nodes.assign((size_t)layouts * nn * 8, 0.0f);
+    for (uint32_t oct = 0; oct < layouts; ++oct) {
         std::vector<float> lay((size_t)nn * 8);
         uint32_t cur = 0;
         emit(B.nodes, 0, oct, lay, cur);

@@ -1,6 +1,7 @@
 // host_common.cpp -- error state, scene layout, device queries, camera C ABI.
 #include "host_common.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include "camera_host.h"
@@ -90,7 +91,9 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
     out->n_light = (uint32_t)l;
 
     HostBvh T;
-    build_bvh(L.sph.data(), (uint32_t)n, &T);
+    uint32_t layouts = kBvhLayouts;
+    if (const char* env = getenv("ATX_BVH_LAYOUTS")) layouts = (uint32_t)atoi(env);
+    build_bvh(L.sph.data(), (uint32_t)n, layouts, &T);
     out->bvh = BvhDev{};
     if (T.n_nodes > 0) {
         ATX_HIP(out->bvh_nodes.reserve(T.nodes.size() * 4 + 16));
@@ -109,6 +112,7 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
         b.big = out->bvh_big.as<int32_t>();
         b.n_nodes = T.n_nodes;
         b.n_big = (uint32_t)T.big.size();
+        b.layout_mask = T.layouts - 1u;
         b.cx = T.cx;
         b.cy = T.cy;
         b.cz = T.cz;

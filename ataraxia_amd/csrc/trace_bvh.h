@@ -104,7 +104,7 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
     float tlim = __builtin_fminf(f.best_hi * inv_a_up, s_up);
 
     const uint32_t n_nodes = B.n_nodes;
-    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    const uint32_t oct = ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) & B.layout_mask;
     const ATX_GLOBAL float4* nodes = (const ATX_GLOBAL float4*)(uintptr_t)B.nodes;
     const ATX_GLOBAL float4* lsph = (const ATX_GLOBAL float4*)(uintptr_t)B.lsph;
     const ATX_GLOBAL int32_t* lidx = (const ATX_GLOBAL int32_t*)(uintptr_t)B.lidx;

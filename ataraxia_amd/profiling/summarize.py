@@ -79,6 +79,15 @@ def main():
             if c in cal:
                 extra = f"  = {mean * cal[c] / 1e6:.3f} MB (calibrated)"
             out.append(f"{short(k, 60):60s} {c:22s} n={n:3d} mean={mean:14.1f}{extra}")
+    p = _db(os.path.join(d, "sq"))
+    if p:
+        c = counters(p)
+        for k in sorted({k for k, _ in c}):
+            tc = c.get((k, "SQ_THREAD_CYCLES_VALU"))
+            ai = c.get((k, "SQ_ACTIVE_INST_VALU"))
+            if tc and ai and ai[0] > 0:
+                out.append(f"{short(k, 60):60s} VALU utilisation (active lanes / 64) = "
+                           f"{100.0 * tc[0] / (ai[0] * 64):.1f}%")
     print("\n".join(out))
 
 

@@ -4,10 +4,11 @@ Default workload (the metric's configuration), ``--workload plain_1080p``:
 1920x1080, 256 random spheres + a ground sphere, 8 point lights, 8 bounces,
 1 sample per pixel per frame, accumulated over the timed frames; synthetic
 scene from the seeded generator.  One "step" is one frame (one path sample
-for every pixel).  `value` is the single-scene renderer's production path
-(the persistent wavefront kernel, several frames per launch); the plain
-one-lane-per-pixel kernel is timed on the same frames and reported beside it.
-Both produce bit-identical images.
+for every pixel).  `value` is the single-scene renderer's production path:
+the persistent wavefront kernel (several frames per launch) with the sphere
+BVH.  The plain one-lane-per-pixel kernel and the flat loop over all spheres
+(the reference's algorithm) are timed on the same frames and reported in
+`variants`.  All four produce bit-identical images.
 
 Other workloads (one JSON line each):
   --workload batch_mix   64 mixed scenes: batched persistent renderer vs the
@@ -16,9 +17,10 @@ Other workloads (one JSON line each):
                          (samples/s, p50/p99 latency, HBM high-water mark)
 
 The roofline entry is the VALU bound of the dominant kernel: 17 algorithmic
-FLOP per ray-sphere test (FMA = 2), tests counted exactly by the kernels,
-divided by that kernel's average launch duration from HIP events recorded on
-the renderer's stream, against MI355X's published FP32 vector peak.
+FLOP per ray-sphere test and 12 per BVH node box test (6 FMA; FMA = 2), both
+counted exactly by the kernels, divided by that kernel's average launch
+duration from HIP events recorded on the renderer's stream, against MI355X's
+published FP32 vector peak.
 """
 import argparse
 import json
@@ -32,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FLOP_PER_TEST = 17
+FLOP_PER_BOX = 12
 
 
 def parse():
@@ -46,7 +49,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time budget of the CPU baseline")
     ap.add_argument("--exact-loop", action="store_true", help="use the exact (slow) closest-hit loop")
     ap.add_argument("--kernel", choices=["plain", "wavefront", "both"], default="both")
+    ap.add_argument("--accel", choices=["bvh", "none", "both"], default="both",
+                    help="sphere BVH, flat loop over all spheres, or both")
     ap.add_argument("--scenes", type=int, default=64, help="batch_mix: number of scenes")
+    ap.add_argument("--tuning", default=None,
+                    help="batch_mix: fixed chunk_px,workgroups,frames_per_launch instead of the autotuner")
+    ap.add_argument("--no-plain", action="store_true", help="batch_mix: skip the plain-kernel comparison")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="batch_mix: scale every scene's spp")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
@@ -91,12 +99,13 @@ def cpu_baseline(sc, target_s):
                       f"in {dt:.2f} s on {threads} threads"}
 
 
-def time_single_scene(sc, kernel, steps, warmup):
+def time_single_scene(sc, kernel, accel, steps, warmup):
     from ataraxia_amd import Renderer
     from ataraxia_amd._lib import lib
     cam = sc.camera
     r = Renderer()
     r.set_kernel(kernel)
+    r.set_accel(accel)
     r.on_resize(cam.width, cam.height)
     r.set_settings(sc.settings)
     r.set_scene(sc)
@@ -113,40 +122,49 @@ def time_single_scene(sc, kernel, steps, warmup):
     return wall, st
 
 
+def algo_flops(st):
+    return FLOP_PER_TEST * st["sphere_tests"] + FLOP_PER_BOX * st["box_tests"]
+
+
 def roofline(st, launches, traffic):
     kernel_ms = st["gpu_ms"] / launches
-    tests = st["sphere_tests"] / launches
-    achieved = FLOP_PER_TEST * tests / (kernel_ms * 1e-3) / 1e12
+    achieved = algo_flops(st) / launches / (kernel_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": kernel_ms,
-            "launches": launches, "sphere_tests_per_launch": tests, "rays_per_launch": st["rays"] / launches,
+            "launches": launches, "sphere_tests_per_launch": st["sphere_tests"] / launches,
+            "box_tests_per_launch": st["box_tests"] / launches, "rays_per_launch": st["rays"] / launches,
             "exact_retraces": st["exact_retraces"]}
 
 
 def bench_plain_1080p(args):
-    from ataraxia_amd import ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, plain_1080p_scene
+    from ataraxia_amd import (ATX_ACCEL_BVH, ATX_ACCEL_NONE, ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT,
+                              plain_1080p_scene)
     sc = plain_1080p_scene()
     cam = sc.camera
     samples = cam.width * cam.height * args.steps
-    res = {}
-    if args.kernel in ("plain", "both"):
-        wall, st = time_single_scene(sc, ATX_KERNEL_PLAIN, args.steps, args.warmup)
-        res["plain"] = (wall, st, roofline(st, args.steps, traffic_from_profile("plain_1080p_k_render_plain.txt")))
-    if args.kernel in ("wavefront", "both"):
-        wall, st = time_single_scene(sc, ATX_KERNEL_WAVEFRONT, args.steps, args.warmup)
-        launches = max(1, -(-args.steps // int(os.environ.get("ATX_WF_FRAMES", "16"))))
-        res["wavefront"] = (wall, st, roofline(st, launches,
-                                               traffic_from_profile("plain_1080p_k_render_wavefront.txt")))
-    main_kernel = "wavefront" if "wavefront" in res else "plain"
-    wall, st, roof = res[main_kernel]
+    kernels = [k for k in ("wavefront", "plain") if args.kernel in (k, "both")]
+    accels = [a for a in ("bvh", "flat") if args.accel in ({"bvh": "bvh", "flat": "none"}[a], "both")]
+    kid = {"plain": ATX_KERNEL_PLAIN, "wavefront": ATX_KERNEL_WAVEFRONT}
+    aid = {"bvh": ATX_ACCEL_BVH, "flat": ATX_ACCEL_NONE}
+    frames_per_launch = int(os.environ.get("ATX_WF_FRAMES", "16"))
+    variants = {}
+    for k in kernels:
+        for a in accels:
+            wall, st = time_single_scene(sc, kid[k], aid[a], args.steps, args.warmup)
+            launches = args.steps if k == "plain" else max(1, -(-args.steps // frames_per_launch))
+            prof = f"plain_1080p_k_render_{k}_{a}.txt"
+            variants[f"{k}/{a}"] = {"value": samples / wall, "ms_per_step": wall * 1e3 / args.steps,
+                                    "roofline": roofline(st, launches, traffic_from_profile(prof))}
+    main_variant = next(iter(variants))   # wavefront/bvh unless restricted
+    v = variants[main_variant]
     out = {
         "metric": "path samples/s (1080p, 8 bounces) + p50/p99 request latency in serve loop",
-        "value": samples / wall,
+        "value": v["value"],
         "unit": "path samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": v["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -154,13 +172,11 @@ def bench_plain_1080p(args):
         "data": "synthetic (seeded splitmix64 scene generator)",
         "config": {"workload": "plain_1080p: 1920x1080, 256 spheres + ground, 8 lights, 8 bounces, "
                                "1 spp/frame accumulated", "renderer": "single-scene",
-                   "kernel": main_kernel, "spheres": int(sc.num_spheres), "lights": int(len(sc.lights)),
-                   "frames": args.steps},
-        "roofline": roof,
+                   "kernel": main_variant.split("/")[0], "accel": main_variant.split("/")[1],
+                   "spheres": int(sc.num_spheres), "lights": int(len(sc.lights)), "frames": args.steps},
+        "roofline": v["roofline"],
+        "variants": variants,
     }
-    if "plain" in res and main_kernel != "plain":
-        pw, pst, proof = res["plain"]
-        out["plain_kernel"] = {"value": samples / pw, "ms_per_step": pw * 1e3 / args.steps, "roofline": proof}
     if not args.no_serve:
         out["serve"] = bench_serve(args)
     if not args.no_cpu_baseline:
@@ -169,19 +185,27 @@ def bench_plain_1080p(args):
 
 
 def bench_batch_mix(args):
-    """Batched persistent renderer vs the plain kernel on the same 64 scenes."""
-    from ataraxia_amd import ATX_KERNEL_PLAIN, Batch, Renderer, batch_mix_scenes
+    """Batched persistent renderer vs the plain kernel on the same 64 scenes
+    (both with the same sphere acceleration)."""
+    from ataraxia_amd import ATX_ACCEL_BVH, ATX_ACCEL_NONE, ATX_KERNEL_PLAIN, Batch, Renderer, batch_mix_scenes
     from ataraxia_amd._lib import lib
+    accel = ATX_ACCEL_NONE if args.accel == "none" else ATX_ACCEL_BVH
     scenes = batch_mix_scenes()[: args.scenes]
     for s in scenes:
         s.spp = max(1, int(round(s.spp * args.spp_scale)))
     samples = sum(s.camera.width * s.camera.height * s.spp for s in scenes)
     b = Batch()
+    b.set_accel(accel)
     for s in scenes:
         b.add_scene(s)
     t0 = time.perf_counter()
-    cache_dir = os.path.join(ROOT, "gpurun_out")
-    tuning = b.autotune(os.path.join(cache_dir, "atx_autotune.txt") if os.path.isdir(cache_dir) else None)
+    if args.tuning:
+        chunk, wg, fpl = (int(v) for v in args.tuning.split(","))
+        b.set_tuning(chunk_px=chunk, workgroups=wg, frames_per_launch=fpl)
+        tuning = {"chunk_px": chunk, "workgroups": wg, "frames_per_launch": fpl, "fixed": True}
+    else:
+        cache_dir = os.path.join(ROOT, "gpurun_out")
+        tuning = b.autotune(os.path.join(cache_dir, "atx_autotune.txt") if os.path.isdir(cache_dir) else None)
     tune_s = time.perf_counter() - t0
     b.render(max_frames=2)       # warm-up: graph capture + first frames
     lib.atx_device_synchronize()
@@ -193,8 +217,9 @@ def bench_batch_mix(args):
     # plain kernel, scene after scene, one launch per frame
     r = Renderer()
     r.set_kernel(ATX_KERNEL_PLAIN)
-    pwall, ptests, pms = 0.0, 0, 0.0
-    for s in scenes:
+    r.set_accel(accel)
+    pwall, pflops, pms = 0.0, 0, 0.0
+    for s in ([] if args.no_plain else scenes):
         r.on_resize(s.camera.width, s.camera.height)
         r.set_settings(s.settings)
         r.set_scene(s)
@@ -205,10 +230,10 @@ def bench_batch_mix(args):
         lib.atx_device_synchronize()
         pwall += time.perf_counter() - t0
         st = r.stats()
-        ptests += st["sphere_tests"]
+        pflops += algo_flops(st)
         pms += st["gpu_ms"]
     r.close()
-    bv, pv = samples / bwall, samples / pwall
+    bv, pv = samples / bwall, (samples / pwall if pwall > 0 else float("nan"))
     return {
         "metric": "path samples/s, batched vs plain on the mixed workload",
         "value": bv, "unit": "path samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -216,11 +241,11 @@ def bench_batch_mix(args):
         "dtype": "f32", "data": "synthetic (seeded splitmix64 scene generator)",
         "config": {"workload": f"batch_mix: {len(scenes)} scenes, 256..1920 x 256..1080, 16..4096 spheres, "
                                f"1..16 lights, 2..10 bounces, spp x{args.spp_scale}", "samples": samples,
-                   "tuning": tuning, "autotune_s": tune_s},
+                   "accel": "flat" if accel == ATX_ACCEL_NONE else "bvh", "tuning": tuning, "autotune_s": tune_s},
         "batched": {"samples_per_s": bv, "wall_s": bwall, "gpu_ms": bst["gpu_ms"],
-                    "tflops": FLOP_PER_TEST * bst["sphere_tests"] / (bst["gpu_ms"] * 1e-3) / 1e12},
+                    "tflops": algo_flops(bst) / (bst["gpu_ms"] * 1e-3) / 1e12},
         "plain": {"samples_per_s": pv, "wall_s": pwall, "gpu_ms": pms,
-                  "tflops": FLOP_PER_TEST * ptests / (pms * 1e-3) / 1e12},
+                  "tflops": pflops / (pms * 1e-3) / 1e12 if pms > 0 else float("nan")},
         "speedup": bv / pv,
     }
 
@@ -239,7 +264,9 @@ def bench_serve(args, rate=None, requests=None):
     for _ in scenes:
         t += -math.log(1.0 - arr.uniform()) / rate
         arrivals.append(t)
-    srv = Server(frames_per_tick=args.frames_per_tick)
+    from ataraxia_amd import ATX_ACCEL_BVH, ATX_ACCEL_NONE
+    srv = Server(frames_per_tick=args.frames_per_tick,
+                 accel=ATX_ACCEL_NONE if args.accel == "none" else ATX_ACCEL_BVH)
     t0 = time.perf_counter()
     ids = []
     for when, sc in zip(arrivals, scenes):

@@ -132,7 +132,8 @@ int32_t trace_bvh(const HostBvh& B, const std::vector<float>& sph, V o, V d, flo
         const float inv_a_up = (1.0f / a) * (1.0f + 0x1p-20f);
         const float s_up = shadow ? std::sqrt(dist2) * (1.0f + 0x1p-18f) : INFINITY;
         float tlim = std::fmin(f.hi * inv_a_up, s_up);
-        const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+        const uint32_t oct = ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) &
+                             (B.layouts - 1u);
         const float* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 8;
         uint32_t i = (shadow && tlim * tlim < dist2) ? B.n_nodes : 0u;
         while (i < B.n_nodes) {
@@ -215,7 +216,7 @@ int check_structure(const HostBvh& B, const std::vector<float>& sph, const char*
     for (int32_t id : B.lidx) seen[id]++;
     for (uint32_t i = 0; i < n_all; ++i)
         if (seen[i] != 1) return fail("sphere not covered exactly once", name);
-    for (uint32_t oct = 0; oct < atx::kBvhLayouts; ++oct) {
+    for (uint32_t oct = 0; oct < B.layouts; ++oct) {
         const float* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 8;
         // walk: a subtree of node i spans [i, skip(i)); its children tile (i, skip)
         std::vector<uint32_t> stack{0};
@@ -342,6 +343,7 @@ V random_dir(Rng& g)
 int main(int argc, char** argv)
 {
     const int rays_per_kind = argc > 1 ? std::atoi(argv[1]) : 20000;
+    const uint32_t layouts = argc > 2 ? (uint32_t)std::atoi(argv[2]) : atx::kBvhLayouts;
     int rc = 0;
     for (Scene& sc : make_scenes()) {
         const uint32_t n = (uint32_t)sc.s.size();
@@ -353,7 +355,7 @@ int main(int argc, char** argv)
             sph[4 * i + 3] = sc.s[i].radius * sc.s[i].radius;
         }
         HostBvh B;
-        atx::build_bvh(sph.data(), n, &B);
+        atx::build_bvh(sph.data(), n, layouts, &B);
         if (B.n_nodes == 0) return fail("no tree built", sc.name);
         if (check_structure(B, sph, sc.name)) return 1;
 

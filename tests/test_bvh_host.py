@@ -30,8 +30,9 @@ def checker(tmp_path_factory):
     return str(exe)
 
 
-def test_bvh_structure_and_culling_match_flat_loop(checker):
-    out = subprocess.run([checker, "64000"], capture_output=True, text=True, timeout=300)
+@pytest.mark.parametrize("layouts", [8, 1])
+def test_bvh_structure_and_culling_match_flat_loop(checker, layouts):
+    out = subprocess.run([checker, "64000", str(layouts)], capture_output=True, text=True, timeout=300)
     print(out.stdout)
     assert out.returncode == 0, out.stderr
     lines = [l for l in out.stdout.splitlines() if "spheres=" in l]

@@ -125,3 +125,30 @@ def test_batch_argument_errors():
         from ataraxia_amd import _lib
         buf = np.zeros(64, np.uint32)
         assert _lib.lib.atx_batch_get_image(b._h, 5, buf.ctypes.data, buf.size) == _lib.ATX_ERR_NOTFOUND
+
+
+def test_autotune_picks_valid_tuning_and_caches_it(tmp_path):
+    """The autotuner returns one of its candidates, records it in the cache
+    file keyed by device and workload, reuses it for the same workload, and
+    the tuned batch still equals the oracle."""
+    scenes = mixed_scenes()
+    cache = tmp_path / "tune.txt"
+    with Batch() as b:
+        ids = [b.add_scene(s) for s in scenes]
+        t1 = b.autotune(str(cache))
+        assert t1["chunk_px"] in (64, 128, 256) and t1["frames_per_launch"] in (16, 32, 64)
+        assert t1["workgroups"] > 0
+        b.render()
+        accs = [b.get_accumulation(i).reshape(-1, 4).copy() for i in ids]
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].split("\t")[1:] == [str(t1["chunk_px"]), str(t1["workgroups"]),
+                                                           str(t1["frames_per_launch"])]
+    with Batch() as b:
+        for s in scenes:
+            b.add_scene(s)
+        t2 = b.autotune(str(cache))      # cache hit: same choice, no new line
+    assert t2 == t1
+    assert len(cache.read_text().splitlines()) == 1
+    for i, s in enumerate(scenes):
+        oacc, _ = oracle_accumulate(s, s.spp)
+        assert np.array_equal(accs[i], oacc), s.name
