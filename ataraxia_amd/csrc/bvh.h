@@ -10,11 +10,11 @@
 // trace_bvh.h for the bound that sizes the per-ray box inflation.
 //
 // Layout in HBM (per scene):
-//   nodes  float4[layouts][n_nodes][2]   (layouts = 1 or 8)
-//          [0] = (bmin.x, bmin.y, bmin.z, skip)   skip: node index after this
-//                                                 node's subtree (n_nodes = end)
-//          [1] = (bmax.x, bmax.y, bmax.z, leaf)   leaf: 0 for an inner node,
-//                                                 else (first << 3) | count
+//   nodes  uint4[layouts][n_nodes]   (layouts = 1 or 8), 16 B per node:
+//          box corners as 16-bit cells of the root box (world = base +
+//          cell * scale per axis), skip (16 bits: node index after this
+//          node's subtree; n_nodes = end) and leaf (16 bits: 0 for an inner
+//          node, else ((first + 1) << 2) | (count - 1)).
 //          Each layout is the same tree in depth-first order; with 8 layouts,
 //          layout k visits, at every inner node, first the child nearer along
 //          the split axis for ray directions whose sign bits are k (x: 1,
@@ -38,7 +38,7 @@
 namespace atx {
 
 struct BvhDev {
-    const float4* nodes;
+    const uint4* nodes;
     const float4* lsph;
     const int32_t* lidx;
     const int32_t* big;
@@ -48,7 +48,10 @@ struct BvhDev {
     float cx, cy, cz;   // centre of the tree's region
     float rb;           // upper bound of |centre - sphere centre| over tree spheres
     float r_min, r_max; // lower / upper bound of the tree spheres' radii
-    float cbox;         // upper bound of |coordinate| over all node boxes
+    float bx, by, bz;   // world position of cell 0
+    float sx, sy, sz;   // world size of one cell
+    float isx, isy, isz;  // 1 / size (rounded)
+    float ext;          // upper bound of 65535 * cell size over the axes
 };
 
 }  // namespace atx

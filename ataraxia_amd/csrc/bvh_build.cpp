@@ -1,9 +1,10 @@
 // bvh_build.cpp -- host construction of the sphere BVH (device layout in bvh.h).
 //
 // Binary tree, surface-area heuristic over sphere boxes, at most kBvhLeafMax
-// spheres per leaf.  All geometry is evaluated in double precision and every
-// box is rounded outwards to float, so a node box contains each of its
-// spheres (centre c, radius sqrt(r*r)) exactly.  Spheres whose radius exceeds
+// spheres per leaf.  All geometry is evaluated in double precision; every box
+// is rounded outwards to float and then to 16-bit cells of the root box (one
+// extra cell on each side), so a node box contains each of its spheres
+// (centre c, radius sqrt(r*r)) exactly.  Spheres whose radius exceeds
 // the half-diagonal of the region of all other centres (a ground sphere of
 // radius 1000 under a cluster of unit spheres) and spheres with non-finite or
 // negative data are kept out of the tree: they would make every box, and
@@ -137,30 +138,45 @@ struct Builder {
     }
 };
 
-void emit(const std::vector<BNode>& T, int id, uint32_t oct, std::vector<float>& out, uint32_t& cur)
+struct Grid {
+    double base[3], scale[3];
+};
+
+uint32_t cell_down(double x, const Grid& g, int a)
+{
+    const double c = std::floor((x - g.base[a]) / g.scale[a]) - 1.0;
+    return (uint32_t)std::min(std::max(c, 0.0), (double)kBvhGrid);
+}
+
+uint32_t cell_up(double x, const Grid& g, int a)
+{
+    const double c = std::ceil((x - g.base[a]) / g.scale[a]) + 1.0;
+    return (uint32_t)std::min(std::max(c, 0.0), (double)kBvhGrid);
+}
+
+void emit(const std::vector<BNode>& T, int id, uint32_t oct, const Grid& g, std::vector<uint32_t>& out,
+          uint32_t& cur)
 {
     const uint32_t me = cur++;
-    float* d = out.data() + (size_t)me * 8;
+    uint32_t* d = out.data() + (size_t)me * 4;
     const BNode& nd = T[id];
     uint32_t skip, leaf;
     if (nd.left < 0) {
         skip = me + 1;
-        leaf = (nd.first << 3) | nd.count;
+        leaf = ((nd.first + 1u) << 2) | (nd.count - 1u);
     } else {
         const bool neg = (oct >> nd.axis) & 1u;
-        emit(T, neg ? nd.right : nd.left, oct, out, cur);
-        emit(T, neg ? nd.left : nd.right, oct, out, cur);
+        emit(T, neg ? nd.right : nd.left, oct, g, out, cur);
+        emit(T, neg ? nd.left : nd.right, oct, g, out, cur);
         skip = cur;
         leaf = 0;
     }
-    d[0] = nd.lo[0];
-    d[1] = nd.lo[1];
-    d[2] = nd.lo[2];
-    std::memcpy(&d[3], &skip, 4);
-    d[4] = nd.hi[0];
-    d[5] = nd.hi[1];
-    d[6] = nd.hi[2];
-    std::memcpy(&d[7], &leaf, 4);
+    const uint32_t lx = cell_down(nd.lo[0], g, 0), ly = cell_down(nd.lo[1], g, 1), lz = cell_down(nd.lo[2], g, 2);
+    const uint32_t hx = cell_up(nd.hi[0], g, 0), hy = cell_up(nd.hi[1], g, 1), hz = cell_up(nd.hi[2], g, 2);
+    d[0] = lx | (ly << 16);
+    d[1] = lz | (hx << 16);
+    d[2] = hy | (hz << 16);
+    d[3] = skip | (leaf << 16);
 }
 
 bool finite_sphere(const float* s)
@@ -229,7 +245,7 @@ void build_bvh(const float* sph4, uint32_t n, uint32_t layouts, HostBvh* out)
         p.idx = (int32_t)i;
         P.push_back(p);
     }
-    if (big.size() > kBvhMaxBig || P.size() < kBvhMinSpheres) return;
+    if (big.size() > kBvhMaxBig || P.size() < kBvhMinSpheres || P.size() > kBvhMaxTree) return;
     std::sort(big.begin(), big.end());
 
     // Region of the tree: centre of the centres' box, bounds of distance and radius.
@@ -255,19 +271,32 @@ void build_bvh(const float* sph4, uint32_t n, uint32_t layouts, HostBvh* out)
     Builder B(P);
     B.build(0, (int)P.size());
     const uint32_t nn = (uint32_t)B.nodes.size();
+
+    // Cell grid over the root box: world = base + cell * scale, with scale a
+    // float so that the device's grid coordinates mean exactly the same boxes.
+    // Boxes round outwards and get one extra cell on each side.
+    Grid g;
+    for (int a = 0; a < 3; ++a) {
+        const double lo_a = B.nodes[0].lo[a], hi_a = B.nodes[0].hi[a];
+        const double span = hi_a - lo_a;
+        float sc = round_up(span > 0.0 ? span / (double)(kBvhGrid - 4) : 1.0);
+        if (!(sc > 0.0f)) sc = 1.0f;
+        g.base[a] = lo_a;
+        g.scale[a] = sc;
+        out->base[a] = B.nodes[0].lo[a];
+        out->scale[a] = sc;
+        out->inv_scale[a] = (float)(1.0 / (double)sc);
+    }
+    out->ext = round_up((double)kBvhGrid * std::max(std::max(g.scale[0], g.scale[1]), g.scale[2]) * (1.0 + 0x1p-30));
     out->n_nodes = nn;
     out->layouts = layouts;
-    out->nodes.assign((size_t)layouts * nn * 8, 0.0f);
+    out->nodes.assign((size_t)layouts * nn * 4, 0u);
     for (uint32_t oct = 0; oct < layouts; ++oct) {
-        std::vector<float> lay((size_t)nn * 8);
+        std::vector<uint32_t> lay((size_t)nn * 4);
         uint32_t cur = 0;
-        emit(B.nodes, 0, oct, lay, cur);
-        std::copy(lay.begin(), lay.end(), out->nodes.begin() + (size_t)oct * nn * 8);
+        emit(B.nodes, 0, oct, g, lay, cur);
+        std::copy(lay.begin(), lay.end(), out->nodes.begin() + (size_t)oct * nn * 4);
     }
-    double cbox = 0.0;
-    for (const BNode& nd : B.nodes)
-        for (int a = 0; a < 3; ++a)
-            cbox = std::max(cbox, std::max(std::fabs((double)nd.lo[a]), std::fabs((double)nd.hi[a])));
     for (const Prim& p : B.leaf_order) {
         for (int a = 0; a < 4; ++a)
             out->lsph.push_back(sph4[4 * p.idx + a]);
@@ -280,13 +309,12 @@ void build_bvh(const float* sph4, uint32_t n, uint32_t layouts, HostBvh* out)
     out->rb = round_up(rb * (1.0 + 0x1p-30));
     out->r_min = round_down(std::sqrt(r2_min) * (1.0 - 0x1p-30));
     out->r_max = round_up(std::sqrt(r2_max) * (1.0 + 0x1p-30));
-    out->cbox = round_up(cbox * (1.0 + 0x1p-30));
 }
 
 size_t bvh_device_bytes(uint64_t n)
 {
     const uint64_t nodes = n > 0 ? 2 * n - 1 : 0;
-    return (size_t)(kBvhLayouts * nodes * 32 + 16) + (size_t)(16 * n + 16) + (size_t)(4 * n + 16) +
+    return (size_t)(kBvhLayouts * nodes * 16 + 16) + (size_t)(16 * n + 16) + (size_t)(4 * n + 16) +
            (size_t)(4 * (kBvhMaxBig + 1) + 16);
 }
 

@@ -106,7 +106,7 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
         if (!T.big.empty())
             ATX_HIP(hipMemcpy(out->bvh_big.get(), T.big.data(), T.big.size() * 4, hipMemcpyHostToDevice));
         BvhDev& b = out->bvh;
-        b.nodes = out->bvh_nodes.as<float4>();
+        b.nodes = out->bvh_nodes.as<uint4>();
         b.lsph = out->bvh_lsph.as<float4>();
         b.lidx = out->bvh_lidx.as<int32_t>();
         b.big = out->bvh_big.as<int32_t>();
@@ -119,7 +119,16 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
         b.rb = T.rb;
         b.r_min = T.r_min;
         b.r_max = T.r_max;
-        b.cbox = T.cbox;
+        b.bx = T.base[0];
+        b.by = T.base[1];
+        b.bz = T.base[2];
+        b.sx = T.scale[0];
+        b.sy = T.scale[1];
+        b.sz = T.scale[2];
+        b.isx = T.inv_scale[0];
+        b.isy = T.inv_scale[1];
+        b.isz = T.inv_scale[2];
+        b.ext = T.ext;
     }
     return ATX_OK;
 }

@@ -13,8 +13,11 @@
 // most 6 D + 2 r).  So every hit the flat loop can report lies in the sphere
 // of radius sqrt(r^2 + 2^-17 (D^2 + r^2)): the box inflation per ray is the
 // largest excess of that radius over r across the tree's radius range
-// (attained at r_min or r_max), plus 2^-19 (|o|_inf + cbox + m) for the
-// rounding of the slab test itself (at most ~5.4 u of those magnitudes).
+// (attained at r_min or r_max), plus a rounding term.  Node boxes are 16-bit
+// cells of the tree's box (world = base + cell * scale); the ray is moved
+// into cell units (o_cell = (o - base) / scale, 1/d scaled by scale), and
+// the rounding term, 2^-18 (|o - base|_inf + 65535 scale_max + m), covers
+// that transform and the slab test (at most ~6 u of those magnitudes).
 //
 // Culling by distance uses an upper bound of the best sphere's exact t
 // (best_hi / a, rounded up by 2^-20): a box whose inflated entry lies beyond
@@ -55,11 +58,12 @@ __device__ __forceinline__ float bvh_inv_dir(float v)
 }
 
 // Radius excess sqrt(r^2 + q) - r, q = 2^-17 (D^2 + r^2), in a form without
-// cancellation.
+// cancellation; hardware sqrt and reciprocal (a few ulp, which the caller's
+// 2^-16 widening covers).
 __device__ __forceinline__ float bvh_excess(float r, float D2)
 {
     const float q = 0x1p-17f * (D2 + r * r);
-    return q / (__builtin_amdgcn_sqrtf(r * r + q) + r);
+    return q * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(r * r + q) + r);
 }
 
 // Closest hit (dist2 < 0) or shadow query (dist2 >= 0) of ray (o, d).
@@ -79,14 +83,15 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
     }
     q.tests += B.n_big;
 
-    // per-ray box inflation
+    // per-ray box inflation (world units), then the ray in cell units
     const float ox = o.x - B.cx, oy = o.y - B.cy, oz = o.z - B.cz;
     const float dist = __builtin_amdgcn_sqrtf((ox * ox + oy * oy) + oz * oz);
     const float D = (dist * (1.0f + 0x1p-18f) + B.rb) * (1.0f + 0x1p-20f);
     const float D2 = D * D;
     const float m = __builtin_fmaxf(bvh_excess(B.r_min, D2), bvh_excess(B.r_max, D2)) * (1.0f + 0x1p-16f);
-    const float omax = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(o.x), __builtin_fabsf(o.y)), __builtin_fabsf(o.z));
-    const float e = 0x1p-19f * ((omax + B.cbox) + m) + 0x1p-60f;
+    const float rx = o.x - B.bx, ry = o.y - B.by, rz = o.z - B.bz;
+    const float omax = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(rx), __builtin_fabsf(ry)), __builtin_fabsf(rz));
+    const float e = 0x1p-18f * ((omax + B.ext) + m) + 0x1p-60f;
     const float M = (m + e) * (1.0f + 0x1p-20f);
     if (!(M < 0x1p60f) || !(omax < 0x1p60f)) {
         // origin too far for the inflation to be formed: flat loop over everything
@@ -96,38 +101,42 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
         q.exact += ex;
         return h;
     }
-    const float ix = bvh_inv_dir(d.x), iy = bvh_inv_dir(d.y), iz = bvh_inv_dir(d.z);
-    const float px = -((o.x + M) * ix), py = -((o.y + M) * iy), pz = -((o.z + M) * iz);
-    const float qx = -((o.x - M) * ix), qy = -((o.y - M) * iy), qz = -((o.z - M) * iz);
+    // t = (cell - o_cell) * (scale / d) along each axis
+    const float ix = bvh_inv_dir(d.x) * B.sx, iy = bvh_inv_dir(d.y) * B.sy, iz = bvh_inv_dir(d.z) * B.sz;
+    const float gx = rx * B.isx, gy = ry * B.isy, gz = rz * B.isz;
+    const float mx = M * B.isx * (1.0f + 0x1p-20f), my = M * B.isy * (1.0f + 0x1p-20f),
+                mz = M * B.isz * (1.0f + 0x1p-20f);
+    const float px = -((gx + mx) * ix), py = -((gy + my) * iy), pz = -((gz + mz) * iz);
+    const float qx = -((gx - mx) * ix), qy = -((gy - my) * iy), qz = -((gz - mz) * iz);
     const float inv_a_up = __builtin_amdgcn_rcpf(a) * (1.0f + 0x1p-20f);
     const float s_up = shadow ? __builtin_amdgcn_sqrtf(dist2) * (1.0f + 0x1p-18f) : __builtin_inff();
     float tlim = __builtin_fminf(f.best_hi * inv_a_up, s_up);
 
     const uint32_t n_nodes = B.n_nodes;
     const uint32_t oct = ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) & B.layout_mask;
-    const ATX_GLOBAL float4* nodes = (const ATX_GLOBAL float4*)(uintptr_t)B.nodes;
+    const ATX_GLOBAL uint4* nodes = (const ATX_GLOBAL uint4*)(uintptr_t)B.nodes + oct * n_nodes;
     const ATX_GLOBAL float4* lsph = (const ATX_GLOBAL float4*)(uintptr_t)B.lsph;
     const ATX_GLOBAL int32_t* lidx = (const ATX_GLOBAL int32_t*)(uintptr_t)B.lidx;
-    const uint32_t base = oct * n_nodes * 2u;
     uint32_t i = (shadow && tlim * tlim < dist2) ? n_nodes : 0u;
     while (i < n_nodes) {
-        const float4 n0 = nodes[base + 2u * i];
-        const float4 n1 = nodes[base + 2u * i + 1u];
+        const uint4 w = nodes[i];
         ++q.boxes;
-        const float ax = __builtin_fmaf(n0.x, ix, px), bx = __builtin_fmaf(n1.x, ix, qx);
-        const float ay = __builtin_fmaf(n0.y, iy, py), by = __builtin_fmaf(n1.y, iy, qy);
-        const float az = __builtin_fmaf(n0.z, iz, pz), bz = __builtin_fmaf(n1.z, iz, qz);
+        const float lx = (float)(w.x & 0xffffu), ly = (float)(w.x >> 16), lz = (float)(w.y & 0xffffu);
+        const float hx = (float)(w.y >> 16), hy = (float)(w.z & 0xffffu), hz = (float)(w.z >> 16);
+        const float ax = __builtin_fmaf(lx, ix, px), bx = __builtin_fmaf(hx, ix, qx);
+        const float ay = __builtin_fmaf(ly, iy, py), by = __builtin_fmaf(hy, iy, qy);
+        const float az = __builtin_fmaf(lz, iz, pz), bz = __builtin_fmaf(hz, iz, qz);
         const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(ax, bx), __builtin_fminf(ay, by)),
                                          __builtin_fmaxf(__builtin_fminf(az, bz), 0.0f));
         const float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
                                          __builtin_fminf(__builtin_fmaxf(az, bz), tlim));
-        uint32_t next = __float_as_uint(n0.w);
+        uint32_t next = w.w & 0xffffu;
         if (tn <= tf) {
-            const uint32_t leaf = __float_as_uint(n1.w);
+            const uint32_t leaf = w.w >> 16;
             if (leaf == 0u) {
                 next = i + 1u;
             } else {
-                const uint32_t first = leaf >> 3, cnt = leaf & 7u;
+                const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
                 for (uint32_t k = 0; k < cnt; ++k) {
                     if (filter_sphere(lsph[first + k], lidx[first + k], o, d, a, f))
                         tlim = __builtin_fminf(tlim, f.best_hi * inv_a_up);

@@ -3,9 +3,9 @@
 // (oracle/atx_oracle.c, ora_trace_ray: the reference's Renderer.cu:251-285).
 //
 // 1. Structure: every layout is a depth-first order of one tree with correct
-//    skip links; child boxes nest in parent boxes; every tree sphere's exact
-//    box lies in its leaf's box; leaves plus the out-of-tree list cover each
-//    sphere exactly once.
+//    skip links; child cell boxes nest in parent boxes; every tree sphere's
+//    exact box lies in its leaf's cell box; leaves plus the out-of-tree list
+//    cover each sphere exactly once.
 // 2. Culling: a float restatement of the traversal in trace_bvh.h (with
 //    correctly rounded sqrt and reciprocal where the device uses the
 //    hardware's approximations) must return the oracle's closest hit (index
@@ -100,7 +100,7 @@ float inv_dir(float v)
 float excess(float r, float D2)
 {
     const float q = 0x1p-17f * (D2 + r * r);
-    return q / (std::sqrt(r * r + q) + r);
+    return q * (1.0f / (std::sqrt(r * r + q) + r));
 }
 
 struct Query {
@@ -121,40 +121,44 @@ int32_t trace_bvh(const HostBvh& B, const std::vector<float>& sph, V o, V d, flo
     const float D = (dist * (1.0f + 0x1p-18f) + B.rb) * (1.0f + 0x1p-20f);
     const float D2 = D * D;
     const float m = std::fmax(excess(B.r_min, D2), excess(B.r_max, D2)) * (1.0f + 0x1p-16f);
-    const float omax = std::fmax(std::fmax(std::fabs(o.x), std::fabs(o.y)), std::fabs(o.z));
-    const float e = 0x1p-19f * ((omax + B.cbox) + m) + 0x1p-60f;
+    const float rx = o.x - B.base[0], ry = o.y - B.base[1], rz = o.z - B.base[2];
+    const float omax = std::fmax(std::fmax(std::fabs(rx), std::fabs(ry)), std::fabs(rz));
+    const float e = 0x1p-18f * ((omax + B.ext) + m) + 0x1p-60f;
     const float M = (m + e) * (1.0f + 0x1p-20f);
     bool flat = !(M < 0x1p60f) || !(omax < 0x1p60f);
     if (!flat) {
-        const float ix = inv_dir(d.x), iy = inv_dir(d.y), iz = inv_dir(d.z);
-        const float px = -((o.x + M) * ix), py = -((o.y + M) * iy), pz = -((o.z + M) * iz);
-        const float qx = -((o.x - M) * ix), qy = -((o.y - M) * iy), qz = -((o.z - M) * iz);
+        const float ix = inv_dir(d.x) * B.scale[0], iy = inv_dir(d.y) * B.scale[1], iz = inv_dir(d.z) * B.scale[2];
+        const float gx = rx * B.inv_scale[0], gy = ry * B.inv_scale[1], gz = rz * B.inv_scale[2];
+        const float mx = M * B.inv_scale[0] * (1.0f + 0x1p-20f), my = M * B.inv_scale[1] * (1.0f + 0x1p-20f),
+                    mz = M * B.inv_scale[2] * (1.0f + 0x1p-20f);
+        const float px = -((gx + mx) * ix), py = -((gy + my) * iy), pz = -((gz + mz) * iz);
+        const float qx = -((gx - mx) * ix), qy = -((gy - my) * iy), qz = -((gz - mz) * iz);
         const float inv_a_up = (1.0f / a) * (1.0f + 0x1p-20f);
         const float s_up = shadow ? std::sqrt(dist2) * (1.0f + 0x1p-18f) : INFINITY;
         float tlim = std::fmin(f.hi * inv_a_up, s_up);
         const uint32_t oct = ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) &
                              (B.layouts - 1u);
-        const float* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 8;
+        const uint32_t* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 4;
         uint32_t i = (shadow && tlim * tlim < dist2) ? B.n_nodes : 0u;
         while (i < B.n_nodes) {
-            const float* n = nodes + (size_t)i * 8;
+            const uint32_t* w = nodes + (size_t)i * 4;
             ++q.boxes;
-            const float ax = std::fma(n[0], ix, px), bx = std::fma(n[4], ix, qx);
-            const float ay = std::fma(n[1], iy, py), by = std::fma(n[5], iy, qy);
-            const float az = std::fma(n[2], iz, pz), bz = std::fma(n[6], iz, qz);
+            const float lx = (float)(w[0] & 0xffffu), ly = (float)(w[0] >> 16), lz = (float)(w[1] & 0xffffu);
+            const float hx = (float)(w[1] >> 16), hy = (float)(w[2] & 0xffffu), hz = (float)(w[2] >> 16);
+            const float ax = std::fma(lx, ix, px), bx = std::fma(hx, ix, qx);
+            const float ay = std::fma(ly, iy, py), by = std::fma(hy, iy, qy);
+            const float az = std::fma(lz, iz, pz), bz = std::fma(hz, iz, qz);
             const float tn = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)),
                                        std::fmax(std::fmin(az, bz), 0.0f));
             const float tf = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
                                        std::fmin(std::fmax(az, bz), tlim));
-            uint32_t skip, leaf;
-            std::memcpy(&skip, &n[3], 4);
-            std::memcpy(&leaf, &n[7], 4);
+            const uint32_t skip = w[3] & 0xffffu, leaf = w[3] >> 16;
             uint32_t next = skip;
             if (tn <= tf) {
                 if (leaf == 0) {
                     next = i + 1;
                 } else {
-                    const uint32_t first = leaf >> 3, cnt = leaf & 7u;
+                    const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
                     for (uint32_t k = 0; k < cnt; ++k)
                         if (filter_sphere(&B.lsph[4 * (first + k)], B.lidx[first + k], o, d, a, f))
                             tlim = std::fmin(tlim, f.hi * inv_a_up);
@@ -200,11 +204,21 @@ int fail(const char* what, const char* scene)
     return 1;
 }
 
-bool box_has_sphere(const float* n, const float* s)
+// Cell box of node word w as world coordinates (double), per the grid.
+void node_box(const HostBvh& B, const uint32_t* w, double lo[3], double hi[3])
+{
+    const uint32_t c[6] = {w[0] & 0xffffu, w[0] >> 16, w[1] & 0xffffu, w[1] >> 16, w[2] & 0xffffu, w[2] >> 16};
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = (double)B.base[a] + (double)c[a] * (double)B.scale[a];
+        hi[a] = (double)B.base[a] + (double)c[3 + a] * (double)B.scale[a];
+    }
+}
+
+bool box_has_sphere(const double lo[3], const double hi[3], const float* s)
 {
     const double r = std::sqrt((double)s[3]);
     for (int a = 0; a < 3; ++a)
-        if ((double)s[a] - r < (double)n[a] || (double)s[a] + r > (double)n[4 + a]) return false;
+        if ((double)s[a] - r < lo[a] || (double)s[a] + r > hi[a]) return false;
     return true;
 }
 
@@ -217,49 +231,43 @@ int check_structure(const HostBvh& B, const std::vector<float>& sph, const char*
     for (uint32_t i = 0; i < n_all; ++i)
         if (seen[i] != 1) return fail("sphere not covered exactly once", name);
     for (uint32_t oct = 0; oct < B.layouts; ++oct) {
-        const float* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 8;
+        const uint32_t* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 4;
         // walk: a subtree of node i spans [i, skip(i)); its children tile (i, skip)
-        std::vector<uint32_t> stack{0};
         uint32_t leaf_spheres = 0;
         for (uint32_t i = 0; i < B.n_nodes; ++i) {
-            const float* n = nodes + (size_t)i * 8;
-            uint32_t skip, leaf;
-            std::memcpy(&skip, &n[3], 4);
-            std::memcpy(&leaf, &n[7], 4);
+            const uint32_t* n = nodes + (size_t)i * 4;
+            const uint32_t skip = n[3] & 0xffffu, leaf = n[3] >> 16;
             if (skip <= i || skip > B.n_nodes) return fail("skip link out of range", name);
             if (leaf != 0 && skip != i + 1) return fail("leaf skip is not the next node", name);
+            double lo[3], hi[3];
+            node_box(B, n, lo, hi);
             if (i > 0) {
                 // parent: the nearest earlier node whose subtree contains i
                 uint32_t p = i - 1;
                 for (;;) {
-                    uint32_t ps;
-                    std::memcpy(&ps, &nodes[(size_t)p * 8 + 3], 4);
-                    if (ps > i) break;
+                    if ((nodes[(size_t)p * 4 + 3] & 0xffffu) > i) break;
                     if (p == 0) return fail("node outside the root's subtree", name);
                     --p;
                 }
-                const float* pn = nodes + (size_t)p * 8;
-                uint32_t ps;
-                std::memcpy(&ps, &pn[3], 4);
-                if (skip > ps) return fail("subtree overlaps its parent's end", name);
+                const uint32_t* pn = nodes + (size_t)p * 4;
+                if (skip > (pn[3] & 0xffffu)) return fail("subtree overlaps its parent's end", name);
+                double plo[3], phi[3];
+                node_box(B, pn, plo, phi);
                 for (int a = 0; a < 3; ++a)
-                    if (n[a] < pn[a] || n[4 + a] > pn[4 + a]) return fail("child box outside parent box", name);
+                    if (lo[a] < plo[a] || hi[a] > phi[a]) return fail("child box outside parent box", name);
             }
             if (leaf != 0) {
-                const uint32_t first = leaf >> 3, cnt = leaf & 7u;
-                if (cnt == 0 || cnt > atx::kBvhLeafMax || first + cnt > B.lidx.size())
-                    return fail("bad leaf range", name);
+                const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
+                if (cnt > atx::kBvhLeafMax || first + cnt > B.lidx.size()) return fail("bad leaf range", name);
                 for (uint32_t k = 0; k < cnt; ++k) {
-                    if (!box_has_sphere(n, &B.lsph[4 * (first + k)])) return fail("sphere outside leaf box", name);
+                    if (!box_has_sphere(lo, hi, &B.lsph[4 * (first + k)])) return fail("sphere outside leaf box", name);
                     if (std::memcmp(&B.lsph[4 * (first + k)], &sph[4 * B.lidx[first + k]], 16) != 0)
                         return fail("leaf sphere differs from scene sphere", name);
                 }
                 leaf_spheres += cnt;
             }
         }
-        uint32_t root_skip;
-        std::memcpy(&root_skip, &nodes[3], 4);
-        if (root_skip != B.n_nodes) return fail("root skip is not the end", name);
+        if ((nodes[3] & 0xffffu) != B.n_nodes) return fail("root skip is not the end", name);
         if (leaf_spheres != B.lidx.size()) return fail("leaves do not hold every tree sphere", name);
     }
     return 0;
