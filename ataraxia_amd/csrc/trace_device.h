@@ -28,8 +28,9 @@
 // whose interval lies wholly at/above the best or at/below zero is rejected by
 // the reference too.  Anything else (overlapping intervals: two spheres within
 // ~1e-5 relative distance along the ray; a sign that the interval cannot
-// decide; disc' below 2^-96 where v_sqrt_f32 is not trusted) marks the lane
-// ambiguous and that lane re-runs trace_exact.  Without ambiguity the winner's
+// decide) marks the lane ambiguous and that lane re-runs trace_exact.  Where
+// v_sqrt_f32 is not trusted (disc' below 2^-96) the interval's half-width is
+// floored at 2^-47, above any error of the square root there.  Without ambiguity the winner's
 // exact t is recomputed once with the exact formula.
 #pragma once
 
@@ -104,20 +105,22 @@ __device__ __forceinline__ Filter filter_init()
 __device__ __forceinline__ bool filter_sphere(float4 s, int32_t id, f3 o, f3 d, float a, Filter& f)
 {
     constexpr float kMargin = 0x1p-17f;
-    constexpr float kTinyDisc = 0x1p-96f;
     const float ocx = o.x - s.x, ocy = o.y - s.y, ocz = o.z - s.z;
     const float b = (ocx * d.x + ocy * d.y) + ocz * d.z;
     const float c = ((ocx * ocx + ocy * ocy) + ocz * ocz) - s.w;
     const float disc = b * b - a * c;
     const float sq = __builtin_amdgcn_sqrtf(disc);  // NaN when disc < 0
     const float nn = -b - sq;
-    const float u = sq + __builtin_fabsf(nn);
+    // The floor keeps the interval sound where v_sqrt_f32 is not trusted
+    // (disc < 2^-96, subnormal inputs): the half-width is then >= 2^-47,
+    // above any error of a square root below 2^-48 plus the subtraction's
+    // rounding.  (disc < 0 gives NaN bounds, which every comparison rejects.)
+    const float u = __builtin_fmaxf(sq + __builtin_fabsf(nn), 0x1p-30f);
     const float hi = __builtin_fmaf(u, kMargin, nn);
     const float lo = __builtin_fmaf(u, -kMargin, nn);
     const bool better = (hi < f.best_lo) & (lo > 0.0f);
     const bool near = (hi > 0.0f) & (lo < f.best_hi);
-    const bool tiny = (disc > 0.0f) & (disc < kTinyDisc);
-    f.amb |= (near & !better) | tiny;
+    f.amb |= near & !better;
     f.best_lo = better ? lo : f.best_lo;
     f.best_hi = better ? hi : f.best_hi;
     f.best = better ? id : f.best;

@@ -67,13 +67,12 @@ bool filter_sphere(const float* s, int32_t id, V o, V d, float a, Filter& f)
     const float disc = b * b - a * c;
     const float sq = std::sqrt(disc);
     const float nn = -b - sq;
-    const float u = sq + std::fabs(nn);
+    const float u = std::fmax(sq + std::fabs(nn), 0x1p-30f);
     const float hi = std::fma(u, 0x1p-17f, nn);
     const float lo = std::fma(u, -0x1p-17f, nn);
     const bool better = (hi < f.lo) && (lo > 0.0f);
     const bool near = (hi > 0.0f) && (lo < f.hi);
-    const bool tiny = (disc > 0.0f) && (disc < 0x1p-96f);
-    f.amb = f.amb || (near && !better) || tiny;
+    f.amb = f.amb || (near && !better);
     if (better) {
         f.lo = lo;
         f.hi = hi;
