@@ -1,7 +1,8 @@
 #!/bin/bash
 # rocprofv3 passes behind profiles/plain_1080p_k_render_<kernel>_<accel>.txt.
 # Run from the repo root on a machine with the GPU:
-#     KERNEL=plain|wavefront ACCEL=bvh|none bash ataraxia_amd/profiling/profile_1080p.sh
+#     KERNEL=plain|wavefront ACCEL=bvh|none [OUT=dir] bash ataraxia_amd/profiling/profile_1080p.sh
+# then summarise with: python ataraxia_amd/profiling/summarize.py <OUT>
 #   1. kernel trace + stats of bench.py (STEPS frames of the 1080p workload)
 #   2. FETCH_SIZE, then WRITE_SIZE, each in its own pass, over one launch:
 #      PMC_STEPS frames (plain: one launch per frame; wavefront: 16 frames
@@ -13,7 +14,7 @@ set -u
 REPO=$(pwd)
 KERNEL=${KERNEL:-wavefront}
 ACCEL=${ACCEL:-bvh}
-OUT=${OUT:-$REPO/gpurun_out/prof_${KERNEL}_$ACCEL}
+OUT=${OUT:-$REPO/profile_out/prof_${KERNEL}_$ACCEL}
 STEPS=${STEPS:-32}
 if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-16}; fi
 BENCH=("$REPO/bench.py" --kernel "$KERNEL" --accel "$ACCEL" --no-serve --no-cpu-baseline)

@@ -27,6 +27,7 @@ import json
 import os
 import re
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -204,8 +205,8 @@ def bench_batch_mix(args):
         b.set_tuning(chunk_px=chunk, workgroups=wg, frames_per_launch=fpl)
         tuning = {"chunk_px": chunk, "workgroups": wg, "frames_per_launch": fpl, "fixed": True}
     else:
-        cache_dir = os.path.join(ROOT, "gpurun_out")
-        tuning = b.autotune(os.path.join(cache_dir, "atx_autotune.txt") if os.path.isdir(cache_dir) else None)
+        cache = os.environ.get("ATX_AUTOTUNE_CACHE", os.path.join(tempfile.gettempdir(), "atx_autotune.txt"))
+        tuning = b.autotune(cache or None)
     tune_s = time.perf_counter() - t0
     b.render(max_frames=2)       # warm-up: graph capture + first frames
     lib.atx_device_synchronize()

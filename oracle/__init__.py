@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (oracle/libatx_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product package
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+compare its result, and by bench.py's cpu_baseline leg.  The product package
 (ataraxia_amd) never imports it.
 """
 from __future__ import annotations

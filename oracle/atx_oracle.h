@@ -1,9 +1,10 @@
 /*
  * atx_oracle.h -- CPU restatement of Ataraxia's per-pixel path tracer.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it may be loaded by
- * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
- * and by nothing else.  The product (ataraxia_amd/) never links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  Results are compared against this library; it
+ * may be loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py, and by nothing else.  The product (ataraxia_amd/) never
+ * links or calls it.
  *
  * Every function restates one reference function in plain scalar C99 with
  * IEEE-754 binary32 arithmetic, no FMA contraction (-ffp-contract=off), and
