@@ -24,8 +24,6 @@
 
 namespace atx {
 
-static bool cur_kahan(atx_batch* h);
-
 static std::string device_key()
 {
     int dev = 0;
@@ -43,10 +41,6 @@ static std::string device_key()
 }  // namespace atx
 
 using namespace atx;
-
-namespace atx {
-static bool cur_kahan(atx_batch* h) { return batch_is_kahan(batch_cast(h)); }
-}  // namespace atx
 
 extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batch_tuning* chosen)
 {
@@ -123,7 +117,7 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
     const uint32_t frames[] = {16u, 32u, 64u};
     for (uint32_t c : chunks) {
         for (uint32_t F : frames) {
-            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(c, cur_kahan(h)), 1, F});
+            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(), 1, F});
             if (rc != ATX_OK) {
                 batch_set_subset(b, {});
                 return rc;

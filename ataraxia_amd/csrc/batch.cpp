@@ -47,7 +47,9 @@ struct Batch {
     std::vector<uint32_t> launch_scenes;  // scenes taking part in launch l
     std::vector<uint32_t> launch_chunks;  // chunk ids of launch l
     uint32_t launches = 0, chunk_px = 0, frames_per_launch = 0, grid = 0, max_spp = 0;
-    bool kahan_lds = false;
+    bool kahan = false;
+    DevBuf d_samples;                     // sample buffer, shared by the launches (stream order)
+    std::vector<uint32_t> launch_px;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
@@ -73,7 +75,7 @@ static int enqueue_launches(Batch* b)
         wp.n_scenes = b->launch_scenes[l];
         wp.total_chunks = b->launch_chunks[l];
         wp.chunk_px = b->chunk_px;
-        wp.kahan = b->kahan_lds ? 1u : 0u;
+        wp.total_px = b->launch_px[l];
         wp.queue = b->d_queue.as<unsigned int>() + l;
         wp.counters = b->counters.as<unsigned long long>();
         ATX_HIP(launch_wavefront(wp, WfLaunch{b->grid}, b->stream));
@@ -97,8 +99,8 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
                      [b](uint32_t x, uint32_t y) { return b->scenes[x]->spp > b->scenes[y]->spp; });
     b->chunk_px = b->tuning.chunk_px ? b->tuning.chunk_px : kDefaultChunkPx;
     b->frames_per_launch = b->tuning.frames_per_launch ? b->tuning.frames_per_launch : kDefaultFramesPerLaunch;
-    b->kahan_lds = b->accumulator == ATX_ACCUM_KAHAN;
-    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid(b->chunk_px, b->kahan_lds);
+    b->kahan = b->accumulator == ATX_ACCUM_KAHAN;
+    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid();
     b->max_spp = 0;
     for (auto& e : b->scenes) b->max_spp = std::max(b->max_spp, std::min(e->spp, max_frames));
     const uint32_t F = b->frames_per_launch;
@@ -106,20 +108,18 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     b->launch_off.clear();
     b->launch_scenes.clear();
     b->launch_chunks.clear();
+    b->launch_px.clear();
     for (uint32_t f0 = 0; f0 < b->max_spp; f0 += F) {
         b->launch_off.push_back((uint32_t)table.size());
-        uint32_t chunks = 0, k = 0;
+        uint32_t k = 0;
         for (; k < n; ++k) {
             BatchEntry& e = *b->scenes[b->order[k]];
             const uint32_t spp = std::min(e.spp, max_frames);
             if (spp <= f0) break;
             WfScene w;
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
-                          b->kahan_lds ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan_lds,
+                          b->kahan ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan,
                           f0 + 1u, std::min(F, spp - f0), b->accel, &w);
-            w.chunk_begin = chunks;
-            w.n_chunks = chunks_for(e.cam.width * e.cam.height, b->chunk_px);
-            chunks += w.n_chunks;
             if (e.capture) {
                 w.dbg_radiance = e.dbg_rad.as<float4>();
                 w.dbg_hit = e.dbg_hit.as<int32_t>();
@@ -129,7 +129,18 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             table.push_back(w);
         }
         b->launch_scenes.push_back(k);
-        b->launch_chunks.push_back(chunks);
+    }
+    // the launches run one after another on the stream: one sample buffer serves all
+    size_t most = 0;
+    for (size_t l = 0; l < b->launch_off.size(); ++l)
+        most = std::max(most, launch_samples(table.data() + b->launch_off[l], b->launch_scenes[l]));
+    ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
+    for (size_t l = 0; l < b->launch_off.size(); ++l) {
+        WfPass lp{};
+        layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], b->chunk_px, b->d_samples.as<float4>(),
+                      &lp);
+        b->launch_chunks.push_back(lp.total_chunks);
+        b->launch_px.push_back(lp.total_px);
     }
     b->launches = (uint32_t)b->launch_off.size();
     ATX_HIP(b->d_table.reserve(std::max<size_t>(1, table.size()) * sizeof(WfScene)));
@@ -346,7 +357,7 @@ int atx_batch_get_tuning(const atx_batch* h, atx_batch_tuning* t)
     if (!t->chunk_px) t->chunk_px = b->chunk_px ? b->chunk_px : kDefaultChunkPx;
     if (!t->frames_per_launch) t->frames_per_launch = kDefaultFramesPerLaunch;
     if (!t->workgroups)
-        t->workgroups = b->grid ? b->grid : wavefront_full_grid(t->chunk_px, b->accumulator == ATX_ACCUM_KAHAN);
+        t->workgroups = b->grid ? b->grid : wavefront_full_grid();
     return ATX_OK;
 }
 

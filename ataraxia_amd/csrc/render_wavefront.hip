@@ -2,27 +2,28 @@
 //
 // The per-pixel path (Renderer.cu:287-387) runs as a per-lane state machine
 // so that all lanes of a wave trace a ray in every iteration:
-//   WAIT    the lane holds sample (pixel, frame) but the pixel's previous
-//           frame is still in flight in another lane
+//   START   begin the path of the lane's (frame, pixel) job
 //   CONT    trace the primary/continuation ray; shade the hit; pick a light
 //   SHADOW  trace the shadow ray; add the light's Cook-Torrance contribution
 //   (then Russian roulette + next direction, or the path ends)
-// A lane whose path ends accumulates its sample into the chunk's LDS
-// accumulator and immediately takes the next sample of the wave's chunk.  The
-// floating-point operations of one path are exactly the plain kernel's (and
-// the oracle's) and each pixel's samples are added in frame order, so the
-// results are bit-identical to rendering each scene alone, one frame per
-// launch, with the reference's per-pixel loop.
+// Every (frame, pixel) path is an independent job.  A lane whose path ends
+// stores the sample in the launch's sample buffer and takes the next job at
+// once, so the work is shared out one path at a time: no lane waits for
+// another pixel's earlier frame and no wave is left with a long tail of
+// serial frames when the queue runs dry.  k_accumulate then adds every
+// pixel's samples in frame order (Renderer.cu:164-168).  The floating-point
+// operations of one path are exactly the plain kernel's (and the oracle's)
+// and the additions happen in the same order, so the results are
+// bit-identical to rendering each scene alone, one frame per launch, with
+// the reference's per-pixel loop.
 //
-// Work unit: a chunk = `chunk_px` consecutive pixels x all frames of the
-// launch.  Each wave keeps two chunk slots in LDS (accumulation, optional
-// Kahan compensation, last finished frame per pixel): while the lanes finish
-// one chunk's last paths the next chunk's samples are already being handed
-// out, so a wave only drains when it changes scene or the work runs out.
-//
-// Spheres are read with wave-uniform indices from global memory, which the
-// compiler lowers to scalar loads (two spheres per s_load_dwordx8, used
-// directly as VALU operands): no LDS staging, no barrier, waves independent.
+// Work distribution: a wave takes blocks of `chunk_px` consecutive pixels of
+// one frame of one scene from the launch's global counter and hands the
+// block's paths to its lanes as they fall idle.  Spheres and the scene table
+// are read with wave-uniform addresses through the constant address space,
+// which the compiler lowers to scalar loads (two spheres per s_load_dwordx8,
+// used directly as VALU operands).  A wave works on one scene at a time and
+// drains its lanes before it switches.
 #include "device_types.h"
 #include "trace_bvh.h"
 #include "trace_device.h"
@@ -31,7 +32,7 @@
 namespace atx {
 namespace {
 
-constexpr uint32_t kIdle = 0, kCont = 1, kShadow = 2, kWait = 3;
+constexpr uint32_t kIdle = 0, kCont = 1, kShadow = 2;
 
 // The scene table and the sphere arrays do not change during a launch; read
 // through the constant address space, wave-uniform loads from them become
@@ -66,104 +67,73 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-// LDS slot of one chunk: acc[P] float4, comp[P] float4 (Kahan only), done[P] u32.
-struct SlotView {
-    float4* acc;
-    float4* comp;
-    uint32_t* done;
-};
-
-__device__ __forceinline__ SlotView slot_view(char* wave_base, uint32_t k, uint32_t P, uint32_t kahan)
-{
-    const uint32_t slot_bytes = P * (20u + 16u * kahan);
-    char* b = wave_base + k * slot_bytes;
-    SlotView v;
-    v.acc = reinterpret_cast<float4*>(b);
-    v.comp = reinterpret_cast<float4*>(b + 16u * P);
-    v.done = reinterpret_cast<uint32_t*>(b + 16u * P * (1u + kahan));
-    return v;
-}
-
 __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
 {
-    extern __shared__ __align__(16) char lds[];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
     const unsigned long long lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const uint32_t P = a.chunk_px;
-    const uint32_t K = a.kahan;
-    char* wave_lds = lds + (size_t)wave * 2u * P * (20u + 16u * K);
 
     // ---- wave-uniform scheduling state ----
     CScene* scenes = as_const(a.scenes);
     int32_t sid = -1;
     CScene* S = nullptr;
-    int32_t pend = -1;
+    int32_t pend = -1;              // a block of another scene, waiting for the lanes to drain
     bool qdone = false;
-    uint32_t cs = 0;                                   // slot whose samples are handed out
-    bool sl_on[2] = {false, false};
-    uint32_t sl_p0[2] = {0, 0}, sl_nv[2] = {0, 0};     // first pixel, valid pixels
-    uint32_t sl_next[2] = {0, 0}, sl_total[2] = {0, 0};
+    uint32_t blk_frame = 0, blk0 = 0, blk_next = 0, blk_end = 0;   // jobs left: pixels [blk0 + next, blk0 + end)
 
     // ---- per-lane path state ----
-    uint32_t state = kIdle, lslot = 0, li = 0, frame = 0;
-    uint32_t px = 0, seed = 0;
+    uint32_t state = kIdle;
+    uint32_t px = 0, frame = 0, seed = 0;
     int32_t bounce = 0;
     f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, tp = ro, col = ro;
     f3 so = ro, nrm = ro, L = ro;   // shadow/next origin, normal, light direction
     float dist2 = 0.0f;
     int32_t mid = 0;                // material of the current hit
     uint32_t lidx = 0;              // light of the current shadow ray
+    bool capture = false;           // this path is the last frame's: write the debug capture
     uint32_t rays = 0, exact = 0;
     unsigned long long tests = 0, boxes = 0;
 
     for (;;) {
-        // ---------------- write back chunks whose samples are all done ----------------
-        for (uint32_t k = 0; k < 2; ++k) {
-            if (!sl_on[k] || sl_next[k] != sl_total[k])
-                continue;
-            if (__any(state != kIdle && lslot == k))
-                continue;
-            const SlotView v = slot_view(wave_lds, k, P, K);
-            const uint32_t flast = S->frame0 + S->frames - 1u;
-            const float fi = (float)flast;
-            for (uint32_t i = lane; i < sl_nv[k]; i += 64u) {
-                const uint32_t q = sl_p0[k] + i;
-                const float4 acc = v.acc[i];
-                S->accum[q] = acc;
-                if (S->kahan) S->comp[q] = v.comp[i];
-                S->image[q] = rgba8(gclamp(acc.x / fi, 0.0f, 1.0f), gclamp(acc.y / fi, 0.0f, 1.0f),
-                                    gclamp(acc.z / fi, 0.0f, 1.0f), gclamp(acc.w / fi, 0.0f, 1.0f));
-            }
-            sl_on[k] = false;
-            __builtin_amdgcn_wave_barrier();
-        }
-
-        // ---------------- refill idle lanes (uniform control flow) ----------------
+        // ---------------- hand jobs to idle lanes (uniform control flow) ----------------
         for (;;) {
             const unsigned long long idle = __ballot(state == kIdle);
             if (idle == 0ull)
                 break;
-            if (sl_on[cs] && sl_next[cs] < sl_total[cs]) {
-                const uint32_t avail = sl_total[cs] - sl_next[cs];
+            if (blk_next < blk_end) {
+                const uint32_t avail = blk_end - blk_next;
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
                 const uint32_t nidle = (uint32_t)__popcll(idle);
                 if (state == kIdle && rank < avail) {
-                    const uint32_t t = sl_next[cs] + rank;
-                    const uint32_t fo = t / sl_nv[cs];
-                    li = t - fo * sl_nv[cs];
-                    frame = S->frame0 + fo;
-                    lslot = cs;
-                    state = kWait;
+                    px = blk0 + blk_next + rank;
+                    frame = blk_frame;
+                    const uint32_t W = S->width;
+                    const uint32_t y = px / W, x = px - y * W;
+                    ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
+                    float ip[16], iv[16];
+                    for (int k = 0; k < 16; ++k) {
+                        ip[k] = S->inv_proj[k];
+                        iv[k] = S->inv_view[k];
+                    }
+                    rd = camera_ray(ip, iv, x, y, W, S->height);
+                    tp = mk(1.0f, 1.0f, 1.0f);
+                    col = mk(0.0f, 0.0f, 0.0f);
+                    seed = x + y * W;
+                    seed *= frame;
+                    bounce = 0;
+                    capture = (S->dbg_hit || S->dbg_shadow) && frame == S->frame0 + S->frames - 1u;
+                    if (capture) {
+                        for (uint32_t i = 0; i < S->dbg_bounces; ++i) {
+                            if (S->dbg_hit) S->dbg_hit[(size_t)px * S->dbg_bounces + i] = -2;
+                            if (S->dbg_shadow) S->dbg_shadow[(size_t)px * S->dbg_bounces + i] = -2;
+                        }
+                    }
+                    state = kCont;
                 }
-                sl_next[cs] += nidle < avail ? nidle : avail;
+                blk_next += nidle < avail ? nidle : avail;
                 continue;
             }
-            // the current slot is exhausted: load the next chunk into a free slot
-            uint32_t fs;
-            if (!sl_on[cs]) fs = cs;
-            else if (!sl_on[cs ^ 1u]) fs = cs ^ 1u;
-            else break;   // both slots in flight: wait for a write-back
+            // the block is used up: take the next one
             int32_t c = pend;
             if (c < 0) {
                 if (qdone)
@@ -182,8 +152,8 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
             while (s + 1 < (int32_t)a.n_scenes && (uint32_t)c >= scenes[s].chunk_begin + scenes[s].n_chunks)
                 ++s;
             if (s != sid) {
-                // a new scene: only once every lane and both slots are drained
-                if (idle != ~0ull || sl_on[0] || sl_on[1]) {
+                // a new scene: only once every lane has finished its path
+                if (idle != ~0ull) {
                     pend = c;
                     break;
                 }
@@ -192,54 +162,15 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
             }
             pend = -1;
             const uint32_t npx = S->width * S->height;
-            const uint32_t p0 = ((uint32_t)c - S->chunk_begin) * P;
-            const uint32_t nv = npx - p0 < P ? npx - p0 : P;
-            const SlotView v = slot_view(wave_lds, fs, P, K);
-            const bool first = S->frame0 == 1u;   // frame 1 starts from zero (the reference's memset)
-            for (uint32_t i = lane; i < nv; i += 64u) {
-                v.acc[i] = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S->accum[p0 + i];
-                if (K) v.comp[i] = (first || !S->kahan) ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S->comp[p0 + i];
-                v.done[i] = S->frame0 - 1u;
-            }
-            __builtin_amdgcn_wave_barrier();
-            sl_on[fs] = true;
-            sl_p0[fs] = p0;
-            sl_nv[fs] = nv;
-            sl_next[fs] = 0;
-            sl_total[fs] = nv * S->frames;
-            cs = fs;
+            const uint32_t per_frame = (npx + P - 1u) / P;
+            const uint32_t rel = (uint32_t)c - S->chunk_begin;
+            const uint32_t fo = rel / per_frame;
+            blk_frame = S->frame0 + fo;
+            blk0 = (rel - fo * per_frame) * P;
+            blk_next = 0;
+            blk_end = npx - blk0 < P ? npx - blk0 : P;
         }
-
-        // ---------------- start samples whose pixel's previous frame is done ----------------
-        if (state == kWait) {
-            const SlotView v = slot_view(wave_lds, lslot, P, K);
-            if (v.done[li] + 1u == frame) {
-                px = sl_p0[0] + li;
-                px = lslot ? sl_p0[1] + li : px;
-                const uint32_t W = S->width;
-                const uint32_t y = px / W, x = px - y * W;
-                ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
-                float ip[16], iv[16];
-                for (int k = 0; k < 16; ++k) {
-                    ip[k] = S->inv_proj[k];
-                    iv[k] = S->inv_view[k];
-                }
-                rd = camera_ray(ip, iv, x, y, W, S->height);
-                tp = mk(1.0f, 1.0f, 1.0f);
-                col = mk(0.0f, 0.0f, 0.0f);
-                seed = x + y * W;
-                seed *= frame;
-                bounce = 0;
-                if (S->dbg_hit || S->dbg_shadow) {
-                    for (uint32_t i = 0; i < S->dbg_bounces; ++i) {
-                        if (S->dbg_hit) S->dbg_hit[(size_t)px * S->dbg_bounces + i] = -2;
-                        if (S->dbg_shadow) S->dbg_shadow[(size_t)px * S->dbg_bounces + i] = -2;
-                    }
-                }
-                state = kCont;
-            }
-        }
-        if (!__any(state != kIdle) && !sl_on[0] && !sl_on[1] && qdone && pend < 0)
+        if (!__any(state != kIdle) && qdone && pend < 0)
             break;
 
         // ---------------- one closest-hit trace for every tracing lane ----------------
@@ -256,7 +187,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                 // the debug capture records the closest occluder; otherwise any proven one will do
                 const BvhDev B = S->bvh;
                 BvhQuery q;
-                const float qd2 = (shadow && S->dbg_shadow == nullptr) ? dist2 : -1.0f;
+                const float qd2 = (shadow && !(capture && S->dbg_shadow)) ? dist2 : -1.0f;
                 h = trace_bvh(B, flat, S->n_sph, o, d, qd2, q);
                 exact += q.exact;
                 tests += q.tests;
@@ -269,7 +200,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
 
             // ---------------- shading (divergent) ----------------
             if (!shadow) {
-                if (S->dbg_hit && (uint32_t)bounce < S->dbg_bounces)
+                if (capture && S->dbg_hit && (uint32_t)bounce < S->dbg_bounces)
                     S->dbg_hit[(size_t)px * S->dbg_bounces + bounce] = h.id;
                 if (h.id < 0) {
                     if (S->sky_light)
@@ -296,7 +227,7 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
                     }
                 }
             } else {
-                if (S->dbg_shadow && (uint32_t)bounce < S->dbg_bounces)
+                if (capture && S->dbg_shadow && (uint32_t)bounce < S->dbg_bounces)
                     S->dbg_shadow[(size_t)px * S->dbg_bounces + bounce] = h.id;
                 const float st = h.id < 0 ? -1.0f : h.t;
                 if (!(st > 0.0f && st * st < dist2)) {
@@ -334,30 +265,12 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
             }
         }
         if (end_path) {
-            // accumulate in frame order (Renderer.cu:164-165) into the chunk's LDS slot
-            const SlotView v = slot_view(wave_lds, lslot, P, K);
-            float4 acc = v.acc[li];
-            if (S->kahan) {
-                float4 cp = v.comp[li];
-                float yv, t;
-                yv = col.x - cp.x; t = acc.x + yv; cp.x = (t - acc.x) - yv; acc.x = t;
-                yv = col.y - cp.y; t = acc.y + yv; cp.y = (t - acc.y) - yv; acc.y = t;
-                yv = col.z - cp.z; t = acc.z + yv; cp.z = (t - acc.z) - yv; acc.z = t;
-                yv = 1.0f - cp.w; t = acc.w + yv; cp.w = (t - acc.w) - yv; acc.w = t;
-                v.comp[li] = cp;
-            } else {
-                acc.x = acc.x + col.x;
-                acc.y = acc.y + col.y;
-                acc.z = acc.z + col.z;
-                acc.w = acc.w + 1.0f;
-            }
-            v.acc[li] = acc;
-            v.done[li] = frame;
-            if (S->dbg_radiance)
+            const uint32_t npx = S->width * S->height;
+            S->samples[(size_t)(frame - S->frame0) * npx + px] = make_float4(col.x, col.y, col.z, 1.0f);
+            if (S->dbg_radiance && frame == S->frame0 + S->frames - 1u)
                 S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
             state = kIdle;
         }
-        __builtin_amdgcn_wave_barrier();
     }
 
     const unsigned long long r = wave_sum(rays);
@@ -372,6 +285,54 @@ __global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
     }
 }
 
+// Adds every pixel's samples of the launch to its accumulation in frame order
+// (Renderer.cu:164-165; frame 1 starts from zero, which is what the
+// reference's memset provides) and writes the RGBA8 value (Renderer.cu:166-168,
+// Renderer.h:70-78).  One lane per pixel of the launch's pixel range.
+__global__ __launch_bounds__(256) void k_accumulate(WfPass a)
+{
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= a.total_px)
+        return;
+    uint32_t lo = 0, hi = a.n_scenes;          // last scene with px_begin <= g
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (a.scenes[mid].px_begin <= g) lo = mid;
+        else hi = mid;
+    }
+    const WfScene& S = a.scenes[lo];
+    const uint32_t npx = S.width * S.height;
+    const uint32_t px = g - S.px_begin;
+    if (px >= npx)
+        return;
+    const bool first = S.frame0 == 1u;
+    float4 acc = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.accum[px];
+    if (S.kahan) {
+        float4 cp = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.comp[px];
+        for (uint32_t f = 0; f < S.frames; ++f) {
+            const float4 c = S.samples[(size_t)f * npx + px];
+            float yv, t;
+            yv = c.x - cp.x; t = acc.x + yv; cp.x = (t - acc.x) - yv; acc.x = t;
+            yv = c.y - cp.y; t = acc.y + yv; cp.y = (t - acc.y) - yv; acc.y = t;
+            yv = c.z - cp.z; t = acc.z + yv; cp.z = (t - acc.z) - yv; acc.z = t;
+            yv = c.w - cp.w; t = acc.w + yv; cp.w = (t - acc.w) - yv; acc.w = t;
+        }
+        S.comp[px] = cp;
+    } else {
+        for (uint32_t f = 0; f < S.frames; ++f) {
+            const float4 c = S.samples[(size_t)f * npx + px];
+            acc.x = acc.x + c.x;
+            acc.y = acc.y + c.y;
+            acc.z = acc.z + c.z;
+            acc.w = acc.w + c.w;
+        }
+    }
+    S.accum[px] = acc;
+    const float fi = (float)(S.frame0 + S.frames - 1u);
+    S.image[px] = rgba8(gclamp(acc.x / fi, 0.0f, 1.0f), gclamp(acc.y / fi, 0.0f, 1.0f),
+                        gclamp(acc.z / fi, 0.0f, 1.0f), gclamp(acc.w / fi, 0.0f, 1.0f));
+}
+
 __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,19 +341,17 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 
 }  // namespace
 
-size_t wavefront_lds_bytes(uint32_t chunk_px, bool kahan)
-{
-    return 4u * 2u * (size_t)chunk_px * (20u + (kahan ? 16u : 0u));
-}
-
 hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream)
 {
     if (p.total_chunks == 0)
         return hipSuccess;
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_render_wavefront, dim3(l.workgroups), dim3(256),
-                       wavefront_lds_bytes(p.chunk_px, p.kahan != 0), stream, p);
+    hipLaunchKernelGGL(k_render_wavefront, dim3(l.workgroups), dim3(256), 0, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_accumulate, dim3((p.total_px + 255u) / 256u), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 
@@ -404,11 +363,10 @@ hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-int wavefront_max_blocks_per_cu(uint32_t chunk_px, bool kahan)
+int wavefront_max_blocks_per_cu()
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront, 256,
-                                                     wavefront_lds_bytes(chunk_px, kahan)) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront, 256, 0) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }

@@ -9,6 +9,7 @@
 //  * no per-frame occupancy/device-property queries or full-device syncs;
 //    work is queued on the handle's own stream and `render` waits on it;
 //  * the RGBA8 image stays on the device until atx_renderer_get_image.
+#include <algorithm>
 #include <new>
 #include <stdlib.h>
 #include <string.h>
@@ -29,7 +30,7 @@ struct Renderer {
     bool has_scene = false;
     int kernel = ATX_KERNEL_PLAIN;
     int accel = ATX_ACCEL_BVH;
-    DevBuf wf_table, wf_queue;
+    DevBuf wf_table, wf_queue, wf_samples;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = kDefaultChunkPx;
     uint32_t wf_frames = kDefaultFramesPerLaunch;
@@ -112,8 +113,6 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             WfScene e;
             fill_wf_scene(r->geo, r->settings, *cam, a.accum, a.comp, a.image, a.comp != nullptr, fi, k, accel,
                           &e);
-            e.chunk_begin = 0;
-            e.n_chunks = chunks_for((uint32_t)npx, chunk);
             e.dbg_radiance = a.dbg_radiance;
             e.dbg_hit = a.dbg_hit;
             e.dbg_shadow = a.dbg_shadow;
@@ -122,6 +121,14 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             launch_frames.push_back(k);
             fi = r->settings.accumulation ? fi + k : 1u;
             left -= k;
+        }
+        // every launch runs after the previous one on the stream: one sample buffer serves all
+        size_t most = 0;
+        for (const WfScene& e : table) most = std::max(most, launch_samples(&e, 1));
+        ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
+        for (WfScene& e : table) {
+            WfPass lp{};
+            layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp);
         }
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
@@ -133,14 +140,14 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
-        const uint32_t grid = wavefront_full_grid(chunk, a.comp != nullptr);
+        const uint32_t grid = wavefront_full_grid();
         for (size_t l = 0; l < table.size(); ++l) {
             WfPass wp{};
             wp.scenes = r->wf_table.as<WfScene>() + l;
             wp.n_scenes = 1;
             wp.total_chunks = table[l].n_chunks;
             wp.chunk_px = chunk;
-            wp.kahan = a.comp != nullptr ? 1u : 0u;
+            wp.total_px = table[l].width * table[l].height;
             wp.queue = r->wf_queue.as<unsigned int>() + l;
             wp.counters = a.counters;
             ATX_HIP(launch_wavefront(wp, WfLaunch{grid}, r->stream));

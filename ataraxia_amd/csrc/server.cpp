@@ -67,6 +67,7 @@ struct Server {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf table, queue_ctr, counters;
+    DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
     uint32_t grid = 0;
     int device = 0;
 };
@@ -76,11 +77,14 @@ static double now_s(const Server* s)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->epoch).count();
 }
 
-static size_t request_bytes(const Request& r, bool kahan)
+// Device bytes a request holds while admitted: scene, tree, accumulation,
+// compensation, image, and its share of the tick's sample buffer.
+static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_tick)
 {
     const size_t npx = (size_t)r.cam.width * r.cam.height;
+    const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
-           npx * (16 + (kahan ? 16 : 0) + 4);
+           npx * (16 + (kahan ? 16 : 0) + 4) + npx * f * 16;
 }
 
 // Device allocations of one request (worker thread).
@@ -170,30 +174,26 @@ static void worker_main(Server* s)
         const uint32_t F = s->cfg.frames_per_tick;
         const uint32_t P = s->cfg.chunk_px;
         table.clear();
-        uint32_t chunks = 0;
         for (auto& r : s->active) {
             WfScene w;
             fill_wf_scene(r->geo, r->settings, r->cam, r->accum.as<float4>(),
                           s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(),
                           s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done),
                           (int)s->cfg.accel, &w);
-            w.chunk_begin = chunks;
-            w.n_chunks = chunks_for(r->cam.width * r->cam.height, P);
-            chunks += w.n_chunks;
             table.push_back(w);
         }
-        bool ok = s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
+        WfPass wp{};
+        bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
+                                     sizeof(float4)) == hipSuccess;
+        if (ok) layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
+        ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
                   hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene),
                                  hipMemcpyHostToDevice, s->stream) == hipSuccess &&
                   launch_zero_u32(s->queue_ctr.as<unsigned int>(), 1, s->stream) == hipSuccess &&
                   hipEventRecord(s->ev0, s->stream) == hipSuccess;
         if (ok) {
-            WfPass wp{};
             wp.scenes = s->table.as<WfScene>();
             wp.n_scenes = (uint32_t)table.size();
-            wp.total_chunks = chunks;
-            wp.chunk_px = P;
-            wp.kahan = s->cfg.kahan;
             wp.queue = s->queue_ctr.as<unsigned int>();
             wp.counters = s->counters.as<unsigned long long>();
             ok = launch_wavefront(wp, WfLaunch{s->grid}, s->stream) == hipSuccess &&
@@ -270,7 +270,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     ATX_HIP(s->queue_ctr.reserve(256));
     ATX_HIP(s->counters.reserve(kCtrCount * sizeof(unsigned long long)));
     ATX_HIP(hipMemset(s->counters.get(), 0, kCtrCount * sizeof(unsigned long long)));
-    s->grid = wavefront_full_grid(s->cfg.chunk_px, s->cfg.kahan != 0);
+    s->grid = wavefront_full_grid();
     s->epoch = std::chrono::steady_clock::now();
     Server* raw = s.release();
     try {
@@ -327,7 +327,7 @@ int atx_server_submit(atx_server* h, const atx_scene_desc* d, uint64_t* request_
     r->settings = d->settings;
     r->cam = d->camera;
     r->spp = d->spp;
-    r->device_bytes = request_bytes(*r, s->cfg.kahan != 0);
+    r->device_bytes = request_bytes(*r, s->cfg.kahan != 0, s->cfg.frames_per_tick);
     std::lock_guard<std::mutex> lk(s->mu);
     s->stats.submitted++;
     if (r->device_bytes > s->cfg.hbm_budget_bytes) {

@@ -3,19 +3,19 @@
 // wavefront mode.
 //
 // One launch renders frames [frame0, frame0 + frames) of every scene in its
-// table.  The launch's work is cut into chunks of `chunk_px` consecutive
-// pixels of one scene; a chunk covers all of the launch's frames of those
-// pixels.  Chunk ids are contiguous per scene (scene s owns [chunk_begin,
-// chunk_begin + n_chunks)).  Waves take chunks from a global atomic counter
-// and hand out the chunk's (frame, pixel) samples frame-major to their lanes,
-// one path at a time: a lane whose path ends immediately starts the next
-// sample, so every closest-hit loop runs with (nearly) all 64 lanes busy.
-// A chunk's accumulation lives in LDS while the chunk is in flight; samples
-// of one pixel are accumulated in frame order (a per-pixel LDS counter holds
-// back a sample until the pixel's previous frame is done), which makes the
-// result bit-identical to rendering one frame per launch.  A wave works on
-// one scene at a time (spheres are read with wave-uniform scalar loads) and
-// switches scenes once its lanes drain.
+// table in two kernels.  The path kernel treats every (frame, pixel) path as
+// an independent job: waves take blocks of `chunk_px` consecutive pixels of
+// one frame of one scene from a global counter (chunk ids are contiguous per
+// scene and frame-major within it: scene s owns [chunk_begin, chunk_begin +
+// n_chunks), n_chunks = frames * ceil(pixels / chunk_px)) and hand the
+// block's paths to their lanes one by one; a lane whose path ends stores its
+// sample in the launch's sample buffer and takes the next job, so every
+// closest-hit loop runs with (nearly) all 64 lanes busy and no wave waits for
+// another's work.  The accumulate kernel then adds each pixel's samples to
+// its accumulation strictly in frame order and writes the RGBA8 value, which
+// makes the result bit-identical to rendering one frame per launch.  A wave
+// works on one scene at a time (spheres are read with wave-uniform scalar
+// loads) and switches scenes once its lanes drain.
 #pragma once
 
 #include <stdint.h>
@@ -41,10 +41,10 @@ struct WfScene {
     uint32_t frame0;       // first frame index rendered by this launch (>= 1)
     uint32_t frames;       // frames rendered by this launch (>= 1)
     uint32_t chunk_begin;  // first chunk id of this scene in the launch
-    uint32_t n_chunks;
+    uint32_t n_chunks;     // frames * ceil(width * height / chunk_px)
     uint32_t kahan;        // 1: compensated accumulation (comp must be valid)
     uint32_t dbg_bounces;  // stride of the debug arrays
-    uint32_t pad_;
+    uint32_t px_begin;     // first pixel of this scene in the launch's pixel range
     float inv_proj[16];
     float inv_view[16];
     float cam_pos[4];
@@ -54,6 +54,7 @@ struct WfScene {
     float4* dbg_radiance;  // optional per-pixel capture of the last frame (nullptr = off)
     int32_t* dbg_hit;
     int32_t* dbg_shadow;
+    float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -62,7 +63,7 @@ struct WfPass {
     uint32_t n_scenes;
     uint32_t total_chunks; // chunk ids [0, total_chunks)
     uint32_t chunk_px;     // pixels per chunk (multiple of 64, <= kMaxChunkPx)
-    uint32_t kahan;        // any scene uses the compensated accumulator (LDS layout)
+    uint32_t total_px;     // pixels of all scenes of the launch
     unsigned int* queue;   // this launch's chunk counter (zero before launch)
     unsigned long long* counters;
 };
@@ -73,11 +74,10 @@ struct WfLaunch {
     uint32_t workgroups;   // persistent grid size (256-lane workgroups)
 };
 
-// LDS bytes per workgroup for a given chunk size / accumulator.
-size_t wavefront_lds_bytes(uint32_t chunk_px, bool kahan);
+// The launch's path kernel, then its accumulate kernel.
 hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream);
 hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
-// Max resident 256-lane workgroups of the wavefront kernel per CU.
-int wavefront_max_blocks_per_cu(uint32_t chunk_px, bool kahan);
+// Max resident 256-lane workgroups of the path kernel per CU.
+int wavefront_max_blocks_per_cu();
 
 }  // namespace atx

@@ -46,6 +46,37 @@ inline uint32_t chunks_for(uint32_t pixels, uint32_t chunk_items)
     return (pixels + chunk_items - 1) / chunk_items;
 }
 
+// Sample-buffer floats4 one launch of these table entries needs.
+inline size_t launch_samples(const WfScene* tab, size_t n)
+{
+    size_t s = 0;
+    for (size_t k = 0; k < n; ++k)
+        s += (size_t)tab[k].width * tab[k].height * tab[k].frames;
+    return s;
+}
+
+// Gives the launch's table entries their chunk ids, pixel offsets and slices
+// of the sample buffer (in table order) and fills the launch's totals.
+inline void layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p)
+{
+    uint32_t chunks = 0, px = 0;
+    size_t off = 0;
+    for (size_t k = 0; k < n; ++k) {
+        WfScene& w = tab[k];
+        const uint32_t npx = w.width * w.height;
+        w.chunk_begin = chunks;
+        w.n_chunks = chunks_for(npx, chunk_px) * w.frames;
+        w.px_begin = px;
+        w.samples = samples + off;
+        chunks += w.n_chunks;
+        px += npx;
+        off += (size_t)npx * w.frames;
+    }
+    p->total_chunks = chunks;
+    p->total_px = px;
+    p->chunk_px = chunk_px;
+}
+
 inline uint32_t device_cus()
 {
     int dev = 0, cus = 0;
@@ -56,9 +87,9 @@ inline uint32_t device_cus()
 }
 
 // Persistent grid that fills the device: CUs x resident workgroups per CU.
-inline uint32_t wavefront_full_grid(uint32_t chunk_px, bool kahan)
+inline uint32_t wavefront_full_grid()
 {
-    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu(chunk_px, kahan);
+    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu();
 }
 
 constexpr uint32_t kDefaultChunkPx = 128;

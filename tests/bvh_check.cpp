@@ -138,32 +138,35 @@ int32_t trace_bvh(const HostBvh& B, const std::vector<float>& sph, V o, V d, flo
         const uint32_t oct = ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) &
                              (B.layouts - 1u);
         const uint32_t* nodes = B.nodes.data() + (size_t)oct * B.n_nodes * 4;
-        uint32_t i = (shadow && tlim * tlim < dist2) ? B.n_nodes : 0u;
-        while (i < B.n_nodes) {
-            const uint32_t* w = nodes + (size_t)i * 4;
-            ++q.boxes;
+        auto slab = [&](const uint32_t* w, float& tn, float& tf) {
             const float lx = (float)(w[0] & 0xffffu), ly = (float)(w[0] >> 16), lz = (float)(w[1] & 0xffffu);
             const float hx = (float)(w[1] >> 16), hy = (float)(w[2] & 0xffffu), hz = (float)(w[2] >> 16);
             const float ax = std::fma(lx, ix, px), bx = std::fma(hx, ix, qx);
             const float ay = std::fma(ly, iy, py), by = std::fma(hy, iy, qy);
             const float az = std::fma(lz, iz, pz), bz = std::fma(hz, iz, qz);
-            const float tn = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)),
-                                       std::fmax(std::fmin(az, bz), 0.0f));
-            const float tf = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
-                                       std::fmin(std::fmax(az, bz), tlim));
-            const uint32_t skip = w[3] & 0xffffu, leaf = w[3] >> 16;
-            uint32_t next = skip;
-            if (tn <= tf) {
-                if (leaf == 0) {
-                    next = i + 1;
-                } else {
-                    const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
-                    for (uint32_t k = 0; k < cnt; ++k)
-                        if (filter_sphere(&B.lsph[4 * (first + k)], B.lidx[first + k], o, d, a, f))
-                            tlim = std::fmin(tlim, f.hi * inv_a_up);
-                    q.tests += cnt;
-                    if (shadow && tlim * tlim < dist2) next = B.n_nodes;
-                }
+            tn = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)), std::fmax(std::fmin(az, bz), 0.0f));
+            tf = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)), std::fmax(az, bz));
+        };
+        auto leaf_spheres = [&](uint32_t leaf) {
+            const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k)
+                if (filter_sphere(&B.lsph[4 * (first + k)], B.lidx[first + k], o, d, a, f))
+                    tlim = std::fmin(tlim, f.hi * inv_a_up);
+            q.tests += cnt;
+        };
+        // one node per step, as in trace_bvh.h
+        uint32_t i = (shadow && tlim * tlim < dist2) ? B.n_nodes : 0u;
+        while (i < B.n_nodes) {
+            const uint32_t* w = nodes + (size_t)i * 4;
+            ++q.boxes;
+            float tn, tf;
+            slab(w, tn, tf);
+            const bool hit = tn <= std::fmin(tf, tlim);
+            const uint32_t leaf = w[3] >> 16;
+            uint32_t next = hit ? i + 1 : (w[3] & 0xffffu);   // a leaf's skip is i + 1
+            if (hit && leaf != 0) {
+                leaf_spheres(leaf);
+                if (shadow && tlim * tlim < dist2) next = B.n_nodes;
             }
             i = next;
         }
