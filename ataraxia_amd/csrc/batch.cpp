@@ -109,13 +109,24 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     b->launch_scenes.clear();
     b->launch_chunks.clear();
     b->launch_px.clear();
+    // Launch (f0, group) renders frames [f0 + 1, f0 + F] of one group of the
+    // scenes whose spp exceeds f0: consecutive scenes in spp order whose
+    // samples fit kSampleBufferBytes (at least one scene per group).
     for (uint32_t f0 = 0; f0 < b->max_spp; f0 += F) {
         b->launch_off.push_back((uint32_t)table.size());
         uint32_t k = 0;
+        size_t group_bytes = 0;
         for (; k < n; ++k) {
             BatchEntry& e = *b->scenes[b->order[k]];
             const uint32_t spp = std::min(e.spp, max_frames);
             if (spp <= f0) break;
+            const size_t bytes = (size_t)e.cam.width * e.cam.height * std::min(F, spp - f0) * sizeof(float4);
+            if (group_bytes > 0 && group_bytes + bytes > kSampleBufferBytes) {
+                b->launch_scenes.push_back((uint32_t)(table.size() - b->launch_off.back()));
+                b->launch_off.push_back((uint32_t)table.size());
+                group_bytes = 0;
+            }
+            group_bytes += bytes;
             WfScene w;
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
                           b->kahan ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan,
@@ -128,7 +139,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             }
             table.push_back(w);
         }
-        b->launch_scenes.push_back(k);
+        b->launch_scenes.push_back((uint32_t)(table.size() - b->launch_off.back()));
     }
     // the launches run one after another on the stream: one sample buffer serves all
     size_t most = 0;
@@ -341,7 +352,7 @@ int atx_batch_set_tuning(atx_batch* h, const atx_batch_tuning* t)
     Batch* b = batch_cast(h);
     ATX_CHECK_ARG(b && t, "NULL argument");
     ATX_CHECK_ARG(t->chunk_px == 0 || (t->chunk_px % 64 == 0 && t->chunk_px <= kMaxChunkPx),
-                  "chunk_px must be 64, 128, 192 or 256");
+                  "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(t->frames_per_launch <= 4096, "frames_per_launch must be <= 4096");
     ATX_CHECK_ARG(t->workgroups <= 65536, "workgroups too large");
     b->tuning = *t;

@@ -108,8 +108,10 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     const bool wf = r->kernel == ATX_KERNEL_WAVEFRONT && frames > 0;
     if (wf) {
         uint32_t fi = r->frame_index, left = frames;
+        const uint32_t fit = (uint32_t)std::max<size_t>(1, kSampleBufferBytes / (npx * sizeof(float4)));
+        const uint32_t per_launch = std::min(r->wf_frames, fit);
         while (left > 0) {
-            const uint32_t k = r->settings.accumulation ? (left < r->wf_frames ? left : r->wf_frames) : 1u;
+            const uint32_t k = r->settings.accumulation ? (left < per_launch ? left : per_launch) : 1u;
             WfScene e;
             fill_wf_scene(r->geo, r->settings, *cam, a.accum, a.comp, a.image, a.comp != nullptr, fi, k, accel,
                           &e);

@@ -253,7 +253,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = 8;
     if (!s->cfg.chunk_px) s->cfg.chunk_px = kDefaultChunkPx;
-    ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be 64..256, step 64");
+    ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");
     s->cfg.kahan = s->cfg.kahan ? 1u : 0u;
     ATX_CHECK_ARG(s->cfg.accel == ATX_ACCEL_BVH || s->cfg.accel == ATX_ACCEL_NONE, "unknown accel %u",

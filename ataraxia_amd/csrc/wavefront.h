@@ -68,7 +68,7 @@ struct WfPass {
     unsigned long long* counters;
 };
 
-constexpr uint32_t kMaxChunkPx = 256;
+constexpr uint32_t kMaxChunkPx = 1024;
 
 struct WfLaunch {
     uint32_t workgroups;   // persistent grid size (256-lane workgroups)

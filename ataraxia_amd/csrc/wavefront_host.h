@@ -92,7 +92,10 @@ inline uint32_t wavefront_full_grid()
     return device_cus() * (uint32_t)wavefront_max_blocks_per_cu();
 }
 
-constexpr uint32_t kDefaultChunkPx = 128;
-constexpr uint32_t kDefaultFramesPerLaunch = 16;
+constexpr uint32_t kDefaultChunkPx = 256;
+constexpr uint32_t kDefaultFramesPerLaunch = 32;
+// Upper bound of one launch's sample buffer (16 B per pixel and frame):
+// renderers shorten their launches, batches split them by scene groups.
+constexpr size_t kSampleBufferBytes = size_t(2) << 30;
 
 }  // namespace atx

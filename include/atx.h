@@ -167,7 +167,7 @@ typedef struct atx_scene_desc {
 } atx_scene_desc;
 
 typedef struct atx_batch_tuning {
-    uint32_t chunk_px;          /* pixels per work chunk: 64, 128, 192 or 256 (0 = 128)   */
+    uint32_t chunk_px;          /* pixels per work chunk: multiple of 64, <= 1024 (0 = 128) */
     uint32_t workgroups;        /* persistent grid, 256-lane workgroups (0 = fill device) */
     uint32_t use_graph;         /* capture all launches in a HIP graph and replay it       */
     uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 16)           */
