@@ -97,7 +97,11 @@ class RequestInfo(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("sphere_tests", C.c_uint64),
-                ("exact_retraces", C.c_uint64), ("box_tests", C.c_uint64), ("gpu_ms", C.c_double)]
+                ("exact_retraces", C.c_uint64), ("box_tests", C.c_uint64), ("gpu_ms", C.c_double),
+                ("launches", C.c_uint64), ("path_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 assert C.sizeof(Sphere) == 20 and C.sizeof(Material) == 52 and C.sizeof(Light) == 28

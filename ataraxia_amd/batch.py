@@ -13,8 +13,7 @@ from .scene import FlatScene
 
 
 def _stats(s: _lib.RenderStats) -> dict:
-    return {"samples": s.samples, "rays": s.rays, "sphere_tests": s.sphere_tests,
-            "exact_retraces": s.exact_retraces, "box_tests": s.box_tests, "gpu_ms": s.gpu_ms}
+    return s.as_dict()
 
 
 class Batch:

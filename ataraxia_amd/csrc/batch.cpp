@@ -212,6 +212,8 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
     b->stats.exact_retraces = ctr[kCtrExact];
     b->stats.box_tests = ctr[kCtrBoxes];
     b->stats.gpu_ms = ms;
+    b->stats.launches = b->launches;
+    b->stats.path_ms = 0.0;
     if (ms_out) *ms_out = ms;
     return ATX_OK;
 }

@@ -341,7 +341,7 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 
 }  // namespace
 
-hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream)
+hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream, hipEvent_t paths_done)
 {
     if (p.total_chunks == 0)
         return hipSuccess;
@@ -349,6 +349,8 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_render_wavefront, dim3(l.workgroups), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && paths_done)
+        e = hipEventRecord(paths_done, stream);
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_accumulate, dim3((p.total_px + 255u) / 256u), dim3(256), 0, stream, p);

@@ -26,6 +26,7 @@ namespace atx {
 struct Renderer {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> wf_ev;   // per wavefront launch: path kernel start, end
     DevScene geo;
     bool has_scene = false;
     int kernel = ATX_KERNEL_PLAIN;
@@ -136,6 +137,11 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
         ATX_HIP(hipMemcpyAsync(r->wf_table.get(), table.data(), table.size() * sizeof(WfScene),
                                hipMemcpyHostToDevice, r->stream));
+        while (r->wf_ev.size() < 2 * table.size()) {
+            hipEvent_t ev = nullptr;
+            ATX_HIP(hipEventCreate(&ev));
+            r->wf_ev.push_back(ev);
+        }
     }
 
     ATX_HIP(hipMemsetAsync(r->counters.get(), 0, kCtrCount * sizeof(unsigned long long), r->stream));
@@ -152,7 +158,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             wp.total_px = table[l].width * table[l].height;
             wp.queue = r->wf_queue.as<unsigned int>() + l;
             wp.counters = a.counters;
-            ATX_HIP(launch_wavefront(wp, WfLaunch{grid}, r->stream));
+            ATX_HIP(hipEventRecord(r->wf_ev[2 * l], r->stream));
+            ATX_HIP(launch_wavefront(wp, WfLaunch{grid}, r->stream, r->wf_ev[2 * l + 1]));
             // Renderer.cu:245-248, once per frame of the launch
             for (uint32_t f = 0; f < launch_frames[l]; ++f) {
                 if (r->settings.accumulation) r->frame_index++;
@@ -177,12 +184,23 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     ATX_HIP(hipStreamSynchronize(r->stream));
     float ms = 0.0f;
     ATX_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    double path_ms = ms;
+    if (wf) {
+        path_ms = 0.0;
+        for (size_t l = 0; l < table.size(); ++l) {
+            float t = 0.0f;
+            ATX_HIP(hipEventElapsedTime(&t, r->wf_ev[2 * l], r->wf_ev[2 * l + 1]));
+            path_ms += t;
+        }
+    }
     r->stats.samples = (uint64_t)npx * frames;
     r->stats.rays = ctr[kCtrRays];
     r->stats.sphere_tests = ctr[kCtrTests];
     r->stats.exact_retraces = ctr[kCtrExact];
     r->stats.box_tests = ctr[kCtrBoxes];
     r->stats.gpu_ms = ms;
+    r->stats.launches = wf ? table.size() : frames;
+    r->stats.path_ms = path_ms;
     return ATX_OK;
 }
 
@@ -233,6 +251,7 @@ int atx_renderer_destroy(atx_renderer* h)
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
+    for (hipEvent_t ev : r->wf_ev) (void)hipEventDestroy(ev);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return ATX_OK;

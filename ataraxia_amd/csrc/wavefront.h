@@ -74,8 +74,10 @@ struct WfLaunch {
     uint32_t workgroups;   // persistent grid size (256-lane workgroups)
 };
 
-// The launch's path kernel, then its accumulate kernel.
-hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream);
+// The launch's path kernel, then its accumulate kernel; `paths_done`, when
+// given, is recorded on the stream between the two.
+hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream,
+                            hipEvent_t paths_done = nullptr);
 hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
 int wavefront_max_blocks_per_cu();

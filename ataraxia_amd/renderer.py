@@ -122,8 +122,7 @@ class Renderer:
     def stats(self) -> dict:
         s = _lib.RenderStats()
         check(lib.atx_renderer_get_stats(self._h, C.byref(s)))
-        return {"samples": s.samples, "rays": s.rays, "sphere_tests": s.sphere_tests,
-                "exact_retraces": s.exact_retraces, "box_tests": s.box_tests, "gpu_ms": s.gpu_ms}
+        return s.as_dict()
 
 
 def camera_ray_directions(camera: Camera) -> np.ndarray:

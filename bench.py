@@ -19,8 +19,10 @@ Other workloads (one JSON line each):
 The roofline entry is the VALU bound of the dominant kernel: 17 algorithmic
 FLOP per ray-sphere test and 12 per BVH node box test (6 FMA; FMA = 2), both
 counted exactly by the kernels, divided by that kernel's average launch
-duration from HIP events recorded on the renderer's stream, against MI355X's
-published FP32 vector peak.
+duration from HIP events recorded on the renderer's stream around the path
+kernel alone (the persistent kernel's launches are each followed by a short
+accumulate kernel, which is timed in `gpu_ms_per_step` but not here), against
+MI355X's published FP32 vector peak.
 """
 import argparse
 import json
@@ -64,13 +66,14 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(name):
-    """HBM bytes per launch of a kernel from a committed rocprof summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+def traffic_from_profile(name, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprof summary
+    (its FETCH_SIZE + WRITE_SIZE lines, calibrated), or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    mb = [float(m) for m in re.findall(r"= ([0-9.]+) MB \(calibrated\)", open(path).read())]
+    mb = [float(m) for line in open(path) if re.search(rf"::{kernel}[(<]", line)
+          for m in re.findall(r"= ([0-9.]+) MB \(calibrated\)", line)]
     return sum(mb) * 1e6 if len(mb) >= 2 else None
 
 
@@ -127,8 +130,9 @@ def algo_flops(st):
     return FLOP_PER_TEST * st["sphere_tests"] + FLOP_PER_BOX * st["box_tests"]
 
 
-def roofline(st, launches, traffic):
-    kernel_ms = st["gpu_ms"] / launches
+def roofline(st, traffic):
+    launches = max(1, st["launches"])
+    kernel_ms = st["path_ms"] / launches
     achieved = algo_flops(st) / launches / (kernel_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": kernel_ms,
@@ -147,15 +151,14 @@ def bench_plain_1080p(args):
     accels = [a for a in ("bvh", "flat") if args.accel in ({"bvh": "bvh", "flat": "none"}[a], "both")]
     kid = {"plain": ATX_KERNEL_PLAIN, "wavefront": ATX_KERNEL_WAVEFRONT}
     aid = {"bvh": ATX_ACCEL_BVH, "flat": ATX_ACCEL_NONE}
-    frames_per_launch = int(os.environ.get("ATX_WF_FRAMES", "16"))
     variants = {}
     for k in kernels:
         for a in accels:
             wall, st = time_single_scene(sc, kid[k], aid[a], args.steps, args.warmup)
-            launches = args.steps if k == "plain" else max(1, -(-args.steps // frames_per_launch))
             prof = f"plain_1080p_k_render_{k}_{a}.txt"
             variants[f"{k}/{a}"] = {"value": samples / wall, "ms_per_step": wall * 1e3 / args.steps,
-                                    "roofline": roofline(st, launches, traffic_from_profile(prof))}
+                                    "gpu_ms_per_step": st["gpu_ms"] / args.steps,
+                                    "roofline": roofline(st, traffic_from_profile(prof, f"k_render_{k}"))}
     main_variant = next(iter(variants))   # wavefront/bvh unless restricted
     v = variants[main_variant]
     out = {

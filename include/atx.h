@@ -96,6 +96,9 @@ typedef struct atx_render_stats {
     uint64_t exact_retraces; /* traces resolved by the exact fallback loop      */
     uint64_t box_tests;      /* ray-box tests of BVH nodes                      */
     double gpu_ms;           /* HIP-event time of the render kernels            */
+    uint64_t launches;       /* path-kernel launches                            */
+    double path_ms;          /* HIP-event time of the path kernel alone, summed
+                                over its launches (renderer; 0 for batches)     */
 } atx_render_stats;
 
 const char* atx_last_error(void);

@@ -112,6 +112,35 @@ def test_wavefront_equals_plain_on_full_1080p():
     assert out[ATX_KERNEL_PLAIN][2] == out[ATX_KERNEL_WAVEFRONT][2]
 
 
+@pytest.mark.parametrize("size,frames,launches", [((48, 30), 70, 3), ((4096, 4096), 20, 3)],
+                         ids=["32-frame-launches", "2GB-sample-cap"])
+def test_wavefront_launch_splits_equal_plain(size, frames, launches):
+    """70 frames split into 32 + 32 + 6 frames per launch, and a 4096x4096
+    image whose 2 GB sample buffer holds 8 frames (8 + 8 + 4): both must
+    match the plain kernel's frame-by-frame accumulation bit for bit."""
+    w, h = size
+    sc = scene_from_camera_overrides(plain_cpu_scene(), w, h)
+    sc.settings = Settings(True, False, 4)
+    out = {}
+    for kernel in (ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT):
+        with Renderer() as r:
+            r.set_kernel(kernel)
+            r.set_accumulator(ATX_ACCUM_KAHAN)
+            r.on_resize(w, h)
+            r.set_settings(sc.settings)
+            r.set_scene(sc)
+            r.render_frames(sc.camera, frames)
+            assert r.frame_index == frames + 1
+            out[kernel] = (r.get_accumulation().copy(), r.get_image().copy(), r.stats())
+    acc_p, img_p, st_p = out[ATX_KERNEL_PLAIN]
+    acc_w, img_w, st_w = out[ATX_KERNEL_WAVEFRONT]
+    assert np.array_equal(acc_p, acc_w)
+    assert np.array_equal(img_p, img_w)
+    assert st_p["launches"] == frames and st_p["path_ms"] == st_p["gpu_ms"]
+    assert st_w["launches"] == launches
+    assert 0.0 < st_w["path_ms"] <= st_w["gpu_ms"]
+
+
 def test_batch_argument_errors():
     from ataraxia_amd import AtxError
     with Batch() as b:
