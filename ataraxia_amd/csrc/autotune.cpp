@@ -117,7 +117,7 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
     const uint32_t frames[] = {16u, 32u, 64u};
     for (uint32_t c : chunks) {
         for (uint32_t F : frames) {
-            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(), 1, F});
+            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(batch_flat_only(b)), 1, F});
             if (rc != ATX_OK) {
                 batch_set_subset(b, {});
                 return rc;

@@ -47,6 +47,7 @@ struct Batch {
     std::vector<uint32_t> launch_scenes;  // scenes taking part in launch l
     std::vector<uint32_t> launch_chunks;  // chunk ids of launch l
     uint32_t launches = 0, chunk_px = 0, frames_per_launch = 0, grid = 0, max_spp = 0;
+    bool flat_only = false;   // no scene has a tree: the flat-loop kernel instance
     bool kahan = false;
     DevBuf d_samples;                     // sample buffer, shared by the launches (stream order)
     std::vector<uint32_t> launch_px;
@@ -78,7 +79,7 @@ static int enqueue_launches(Batch* b)
         wp.total_px = b->launch_px[l];
         wp.queue = b->d_queue.as<unsigned int>() + l;
         wp.counters = b->counters.as<unsigned long long>();
-        ATX_HIP(launch_wavefront(wp, WfLaunch{b->grid}, b->stream));
+        ATX_HIP(launch_wavefront(wp, WfLaunch{b->grid, b->flat_only}, b->stream));
     }
     return ATX_OK;
 }
@@ -100,7 +101,8 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     b->chunk_px = b->tuning.chunk_px ? b->tuning.chunk_px : kDefaultChunkPx;
     b->frames_per_launch = b->tuning.frames_per_launch ? b->tuning.frames_per_launch : kDefaultFramesPerLaunch;
     b->kahan = b->accumulator == ATX_ACCUM_KAHAN;
-    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid();
+    b->flat_only = batch_flat_only(b);
+    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid(b->flat_only);
     b->max_spp = 0;
     for (auto& e : b->scenes) b->max_spp = std::max(b->max_spp, std::min(e->spp, max_frames));
     const uint32_t F = b->frames_per_launch;
@@ -223,6 +225,14 @@ Batch* batch_cast(atx_batch* h) { return reinterpret_cast<Batch*>(h); }
 bool batch_is_kahan(const Batch* b) { return b->accumulator == ATX_ACCUM_KAHAN; }
 
 int batch_accel(const Batch* b) { return b->accel; }
+
+bool batch_flat_only(const Batch* b)
+{
+    for (const auto& e : b->scenes)
+        if (e->geo.accel(b->accel).n_nodes != 0)
+            return false;
+    return true;
+}
 
 void batch_set_subset(Batch* b, const std::vector<uint32_t>& ids)
 {
@@ -370,7 +380,7 @@ int atx_batch_get_tuning(const atx_batch* h, atx_batch_tuning* t)
     if (!t->chunk_px) t->chunk_px = b->chunk_px ? b->chunk_px : kDefaultChunkPx;
     if (!t->frames_per_launch) t->frames_per_launch = kDefaultFramesPerLaunch;
     if (!t->workgroups)
-        t->workgroups = b->grid ? b->grid : wavefront_full_grid();
+        t->workgroups = b->grid ? b->grid : wavefront_full_grid(batch_flat_only(b));
     return ATX_OK;
 }
 

@@ -13,6 +13,8 @@ namespace atx {
 struct Batch;
 Batch* batch_cast(atx_batch* h);
 int batch_accel(const Batch* b);
+// True when no scene has a tree with the batch's acceleration setting.
+bool batch_flat_only(const Batch* b);
 uint64_t batch_scene_pixels(const Batch* b, uint32_t id);
 // Renders each scene's first max_frames frames; *ms_out gets the HIP-event time.
 int batch_run(Batch* b, uint32_t max_frames, double* ms_out);

@@ -24,6 +24,12 @@
 // which the compiler lowers to scalar loads (two spheres per s_load_dwordx8,
 // used directly as VALU operands).  A wave works on one scene at a time and
 // drains its lanes before it switches.
+//
+// Registers: the kernel is compiled twice, for 6 and for 7 resident waves
+// per SIMD (amdgpu_waves_per_eu: at most 80 and 72 VGPRs, a few spilled to
+// scratch).  More waves hide more latency than the spills cost: launches that
+// traverse trees take the 6-wave instance, launches whose scenes all use the
+// flat loop the 7-wave one (the natural allocation, 93 VGPRs, allows 5).
 #include "device_types.h"
 #include "trace_bvh.h"
 #include "trace_device.h"
@@ -67,7 +73,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_render_wavefront(WfPass a)
+template <uint32_t kWaves>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_render_wavefront(WfPass a)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
@@ -347,7 +354,10 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
         return hipSuccess;
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_render_wavefront, dim3(l.workgroups), dim3(256), 0, stream, p);
+    if (l.flat_only)
+        hipLaunchKernelGGL(k_render_wavefront<kWfWavesFlat>, dim3(l.workgroups), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_render_wavefront<kWfWavesTree>, dim3(l.workgroups), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && paths_done)
         e = hipEventRecord(paths_done, stream);
@@ -365,10 +375,13 @@ hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-int wavefront_max_blocks_per_cu()
+int wavefront_max_blocks_per_cu(bool flat_only)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront, 256, 0) != hipSuccess)
+    const hipError_t e =
+        flat_only ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesFlat>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesTree>, 256, 0);
+    if (e != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }

@@ -148,7 +148,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
-        const uint32_t grid = wavefront_full_grid();
+        const bool flat = wf_flat_only(table.data(), table.size());
+        const WfLaunch shape{wavefront_full_grid(flat), flat};
         for (size_t l = 0; l < table.size(); ++l) {
             WfPass wp{};
             wp.scenes = r->wf_table.as<WfScene>() + l;
@@ -159,7 +160,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             wp.queue = r->wf_queue.as<unsigned int>() + l;
             wp.counters = a.counters;
             ATX_HIP(hipEventRecord(r->wf_ev[2 * l], r->stream));
-            ATX_HIP(launch_wavefront(wp, WfLaunch{grid}, r->stream, r->wf_ev[2 * l + 1]));
+            ATX_HIP(launch_wavefront(wp, shape, r->stream, r->wf_ev[2 * l + 1]));
             // Renderer.cu:245-248, once per frame of the launch
             for (uint32_t f = 0; f < launch_frames[l]; ++f) {
                 if (r->settings.accumulation) r->frame_index++;

@@ -68,7 +68,7 @@ struct Server {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
-    uint32_t grid = 0;
+    uint32_t grid = 0, grid_flat = 0;   // persistent grids: ticks with trees, flat-loop-only ticks
     int device = 0;
 };
 
@@ -196,7 +196,8 @@ static void worker_main(Server* s)
             wp.n_scenes = (uint32_t)table.size();
             wp.queue = s->queue_ctr.as<unsigned int>();
             wp.counters = s->counters.as<unsigned long long>();
-            ok = launch_wavefront(wp, WfLaunch{s->grid}, s->stream) == hipSuccess &&
+            const bool flat = wf_flat_only(table.data(), table.size());
+            ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
                  hipEventRecord(s->ev1, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
         }
         float ms = 0.0f;
@@ -270,7 +271,8 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     ATX_HIP(s->queue_ctr.reserve(256));
     ATX_HIP(s->counters.reserve(kCtrCount * sizeof(unsigned long long)));
     ATX_HIP(hipMemset(s->counters.get(), 0, kCtrCount * sizeof(unsigned long long)));
-    s->grid = wavefront_full_grid();
+    s->grid = wavefront_full_grid(false);
+    s->grid_flat = wavefront_full_grid(true);
     s->epoch = std::chrono::steady_clock::now();
     Server* raw = s.release();
     try {

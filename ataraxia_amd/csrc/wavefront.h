@@ -70,8 +70,14 @@ struct WfPass {
 
 constexpr uint32_t kMaxChunkPx = 1024;
 
+// Resident waves per SIMD the path kernel's registers are budgeted for:
+// launches with trees, and launches whose scenes all use the flat loop.
+constexpr uint32_t kWfWavesTree = 6;
+constexpr uint32_t kWfWavesFlat = 7;
+
 struct WfLaunch {
-    uint32_t workgroups;   // persistent grid size (256-lane workgroups)
+    uint32_t workgroups;     // persistent grid size (256-lane workgroups)
+    bool flat_only = false;  // no scene of the launch has a tree
 };
 
 // The launch's path kernel, then its accumulate kernel; `paths_done`, when
@@ -80,6 +86,6 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
                             hipEvent_t paths_done = nullptr);
 hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
-int wavefront_max_blocks_per_cu();
+int wavefront_max_blocks_per_cu(bool flat_only);
 
 }  // namespace atx

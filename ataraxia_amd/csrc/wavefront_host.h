@@ -87,9 +87,18 @@ inline uint32_t device_cus()
 }
 
 // Persistent grid that fills the device: CUs x resident workgroups per CU.
-inline uint32_t wavefront_full_grid()
+inline uint32_t wavefront_full_grid(bool flat_only)
 {
-    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu();
+    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu(flat_only);
+}
+
+// True when no scene of the table has a tree (the flat-loop kernel instance).
+inline bool wf_flat_only(const WfScene* tab, size_t n)
+{
+    for (size_t k = 0; k < n; ++k)
+        if (tab[k].bvh.n_nodes != 0)
+            return false;
+    return true;
 }
 
 constexpr uint32_t kDefaultChunkPx = 256;
