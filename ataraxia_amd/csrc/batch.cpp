@@ -26,6 +26,8 @@ namespace atx {
 struct BatchEntry {
     DevScene geo;
     DevBuf accum, comp, image;
+    DevBuf dirs;                  // primary ray direction per pixel (written once)
+    bool dirs_ready = false;
     atx_settings settings{};
     atx_camera cam{};
     uint32_t spp = 1;
@@ -133,6 +135,12 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
                           b->kahan ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan,
                           f0 + 1u, std::min(F, spp - f0), b->accel, &w);
+            if (!e.dirs_ready) {
+                ATX_HIP(e.dirs.reserve((size_t)e.cam.width * e.cam.height * sizeof(float4)));
+                ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
+                e.dirs_ready = true;
+            }
+            w.dirs = e.dirs.as<float4>();
             if (e.capture) {
                 w.dbg_radiance = e.dbg_rad.as<float4>();
                 w.dbg_hit = e.dbg_hit.as<int32_t>();

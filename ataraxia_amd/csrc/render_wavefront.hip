@@ -114,18 +114,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 if (state == kIdle && rank < avail) {
                     px = blk0 + blk_next + rank;
                     frame = blk_frame;
-                    const uint32_t W = S->width;
-                    const uint32_t y = px / W, x = px - y * W;
                     ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
-                    float ip[16], iv[16];
-                    for (int k = 0; k < 16; ++k) {
-                        ip[k] = S->inv_proj[k];
-                        iv[k] = S->inv_view[k];
+                    if (S->dirs) {
+                        const float4 dv = S->dirs[px];
+                        rd = mk(dv.x, dv.y, dv.z);
+                    } else {
+                        const uint32_t W = S->width;
+                        const uint32_t y = px / W, x = px - y * W;
+                        float ip[16], iv[16];
+                        for (int k = 0; k < 16; ++k) {
+                            ip[k] = S->inv_proj[k];
+                            iv[k] = S->inv_view[k];
+                        }
+                        rd = camera_ray(ip, iv, x, y, W, S->height);
                     }
-                    rd = camera_ray(ip, iv, x, y, W, S->height);
                     tp = mk(1.0f, 1.0f, 1.0f);
                     col = mk(0.0f, 0.0f, 0.0f);
-                    seed = x + y * W;
+                    seed = px;   // x + y * W
                     seed *= frame;
                     bounce = 0;
                     capture = (S->dbg_hit || S->dbg_shadow) && frame == S->frame0 + S->frames - 1u;
@@ -340,6 +345,23 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
                         gclamp(acc.z / fi, 0.0f, 1.0f), gclamp(acc.w / fi, 0.0f, 1.0f));
 }
 
+struct CamDirArgs {
+    float inv_proj[16];
+    float inv_view[16];
+    uint32_t width, height;
+    float4* dirs;
+};
+
+__global__ __launch_bounds__(256) void k_camera_dirs(CamDirArgs c)
+{
+    const uint32_t px = blockIdx.x * 256u + threadIdx.x;
+    if (px >= c.width * c.height)
+        return;
+    const uint32_t y = px / c.width, x = px - y * c.width;
+    const f3 d = camera_ray(c.inv_proj, c.inv_view, x, y, c.width, c.height);
+    c.dirs[px] = make_float4(d.x, d.y, d.z, 0.0f);
+}
+
 __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,6 +386,23 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_accumulate, dim3((p.total_px + 255u) / 256u), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream)
+{
+    const uint32_t n = s.width * s.height;
+    if (n == 0)
+        return hipSuccess;
+    CamDirArgs c;
+    for (int k = 0; k < 16; ++k) {
+        c.inv_proj[k] = s.inv_proj[k];
+        c.inv_view[k] = s.inv_view[k];
+    }
+    c.width = s.width;
+    c.height = s.height;
+    c.dirs = dirs;
+    hipLaunchKernelGGL(k_camera_dirs, dim3((n + 255u) / 256u), dim3(256), 0, stream, c);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ struct Renderer {
     bool has_scene = false;
     int kernel = ATX_KERNEL_PLAIN;
     int accel = ATX_ACCEL_BVH;
-    DevBuf wf_table, wf_queue, wf_samples;
+    DevBuf wf_table, wf_queue, wf_samples, wf_dirs;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = kDefaultChunkPx;
     uint32_t wf_frames = kDefaultFramesPerLaunch;
@@ -133,6 +133,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             WfPass lp{};
             layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp);
         }
+        ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
+        for (WfScene& e : table) e.dirs = r->wf_dirs.as<float4>();
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
         ATX_HIP(hipMemcpyAsync(r->wf_table.get(), table.data(), table.size() * sizeof(WfScene),
@@ -148,6 +150,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
+        // the camera is the same for every launch of this call
+        ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
         const bool flat = wf_flat_only(table.data(), table.size());
         const WfLaunch shape{wavefront_full_grid(flat), flat};
         for (size_t l = 0; l < table.size(); ++l) {

@@ -42,6 +42,7 @@ struct Request {
     // device state while admitted
     DevScene geo;
     DevBuf accum, comp, image;
+    DevBuf dirs;                    // primary ray direction per pixel
     uint32_t frames_done = 0;
     // results
     std::vector<uint32_t> rgba;
@@ -78,13 +79,14 @@ static double now_s(const Server* s)
 }
 
 // Device bytes a request holds while admitted: scene, tree, accumulation,
-// compensation, image, and its share of the tick's sample buffer.
+// compensation, image, primary ray directions, and its share of the tick's
+// sample buffer.
 static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_tick)
 {
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
-           npx * (16 + (kahan ? 16 : 0) + 4) + npx * f * 16;
+           npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16;
 }
 
 // Device allocations of one request (worker thread).
@@ -94,14 +96,25 @@ static bool admit(Server* s, Request& r)
                           r.lights.data(), r.lights.size(), &r.geo);
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     if (rc == ATX_OK && (r.accum.reserve(npx * 16) != hipSuccess || r.image.reserve(npx * 4) != hipSuccess ||
+                         r.dirs.reserve(npx * 16) != hipSuccess ||
                          (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
         rc = ATX_ERR_NOMEM;
+    if (rc == ATX_OK) {
+        // on the worker's stream, ahead of the request's first launch
+        WfScene w;
+        fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, ATX_ACCEL_NONE, &w);
+        if (launch_camera_dirs(w, r.dirs.as<float4>(), s->stream) != hipSuccess) {
+            set_error("primary ray directions: launch failed");
+            rc = ATX_ERR_HIP;
+        }
+    }
     if (rc != ATX_OK) {
         r.error = atx_last_error();
         r.geo = DevScene();
         r.accum.release();
         r.comp.release();
         r.image.release();
+        r.dirs.release();
         return false;
     }
     return true;
@@ -123,6 +136,7 @@ static void finish(Server* s, Request& r, bool ok)
     r.accum.release();
     r.comp.release();
     r.image.release();
+    r.dirs.release();
     std::lock_guard<std::mutex> lk(s->mu);
     r.state = ok ? ATX_REQ_DONE : ATX_REQ_FAILED;
     r.t_done = now_s(s);
@@ -180,6 +194,7 @@ static void worker_main(Server* s)
                           s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(),
                           s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done),
                           (int)s->cfg.accel, &w);
+            w.dirs = r->dirs.as<float4>();
             table.push_back(w);
         }
         WfPass wp{};

@@ -55,6 +55,7 @@ struct WfScene {
     int32_t* dbg_hit;
     int32_t* dbg_shadow;
     float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
+    const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -85,6 +86,9 @@ struct WfLaunch {
 hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream,
                             hipEvent_t paths_done = nullptr);
 hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
+// Writes the primary ray direction of every pixel of s's camera to dirs
+// (width * height float4, w = 0): camera_ray, computed once instead of per path.
+hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
 int wavefront_max_blocks_per_cu(bool flat_only);
 
