@@ -26,8 +26,8 @@
 // drains its lanes before it switches.
 //
 // Registers: the kernel is compiled twice, for 6 and for 7 resident waves
-// per SIMD (amdgpu_waves_per_eu: at most 80 and 72 VGPRs, a few spilled to
-// scratch).  More waves hide more latency than the spills cost: launches that
+// per SIMD (amdgpu_waves_per_eu: at most 80 and 72 VGPRs, 2 and 16 spilled
+// to scratch).  More waves hide more latency than the spills cost: launches that
 // traverse trees take the 6-wave instance, launches whose scenes all use the
 // flat loop the 7-wave one (the natural allocation, 93 VGPRs, allows 5).
 #include "device_types.h"
@@ -73,6 +73,20 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// Adds the wave's per-lane test counts to the launch's counters and clears
+// them (every lane of the wave calls it).
+__device__ __forceinline__ void flush_counters(unsigned long long* ctr, uint32_t lane, uint32_t& tests,
+                                               uint32_t& boxes)
+{
+    const unsigned long long ts = wave_sum(tests);
+    const unsigned long long bx = wave_sum(boxes);
+    if (lane == 0) {
+        if (ts) atomicAdd(ctr + kCtrTests, ts);
+        if (bx) atomicAdd(ctr + kCtrBoxes, bx);
+    }
+    tests = boxes = 0u;
+}
+
 template <uint32_t kWaves>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_render_wavefront(WfPass a)
 {
@@ -98,8 +112,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     int32_t mid = 0;                // material of the current hit
     uint32_t lidx = 0;              // light of the current shadow ray
     bool capture = false;           // this path is the last frame's: write the debug capture
-    uint32_t rays = 0, exact = 0;
-    unsigned long long tests = 0, boxes = 0;
+    // statistics: rays per wave; test counts per lane, added to the launch's
+    // 64-bit counters at the end (and earlier if one could overflow); exact
+    // retraces (rare) straight to their counter
+    unsigned long long rays = 0;
+    uint32_t tests = 0, boxes = 0;
 
     for (;;) {
         // ---------------- hand jobs to idle lanes (uniform control flow) ----------------
@@ -189,7 +206,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         const bool tracing = state == kCont || state == kShadow;
         bool end_bounce = false;   // run RR + next direction
         bool end_path = state == kCont && S->max_bounces <= 0;   // zero bounces: black sample
-        if (tracing && !end_path) {
+        const bool traces = tracing && !end_path;
+        rays += (unsigned long long)__popcll(__ballot(traces));
+        if (traces) {
             const bool shadow = state == kShadow;
             const f3 o = shadow ? so : ro;
             const f3 d = shadow ? L : rd;
@@ -201,14 +220,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 BvhQuery q;
                 const float qd2 = (shadow && !(capture && S->dbg_shadow)) ? dist2 : -1.0f;
                 h = trace_bvh(B, flat, S->n_sph, o, d, qd2, q);
-                exact += q.exact;
+                if (q.exact)
+                    atomicAdd(a.counters + kCtrExact, (unsigned long long)q.exact);
                 tests += q.tests;
                 boxes += q.boxes;
             } else {
-                h = trace_filtered(flat, S->n_sph, o, d, exact);
+                uint32_t ex = 0;
+                h = trace_filtered(flat, S->n_sph, o, d, ex);
+                if (ex)
+                    atomicAdd(a.counters + kCtrExact, (unsigned long long)ex);
                 tests += S->n_sph;
             }
-            ++rays;
 
             // ---------------- shading (divergent) ----------------
             if (!shadow) {
@@ -276,6 +298,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 }
             }
         }
+        if (__any(tests >= 0x80000000u || boxes >= 0x80000000u))
+            flush_counters(a.counters, lane, tests, boxes);
         if (end_path) {
             const uint32_t npx = S->width * S->height;
             S->samples[(size_t)(frame - S->frame0) * npx + px] = make_float4(col.x, col.y, col.z, 1.0f);
@@ -285,16 +309,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         }
     }
 
-    const unsigned long long r = wave_sum(rays);
-    const unsigned long long e = wave_sum(exact);
-    const unsigned long long ts = wave_sum(tests);
-    const unsigned long long bx = wave_sum(boxes);
-    if (lane == 0) {
-        if (r) atomicAdd(a.counters + kCtrRays, r);
-        if (e) atomicAdd(a.counters + kCtrExact, e);
-        if (ts) atomicAdd(a.counters + kCtrTests, ts);
-        if (bx) atomicAdd(a.counters + kCtrBoxes, bx);
-    }
+    flush_counters(a.counters, lane, tests, boxes);
+    if (lane == 0 && rays)
+        atomicAdd(a.counters + kCtrRays, rays);
 }
 
 // Adds every pixel's samples of the launch to its accumulation in frame order
