@@ -13,8 +13,8 @@ import pytest
 import oracle as O
 from ataraxia_amd import (ATX_ACCUM_KAHAN, ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Batch, Camera, Renderer,
                           Settings, default_scene, plain_1080p_scene, plain_cpu_scene)
-from ataraxia_amd.scene import (FlatScene, SplitMix64, make_lights, make_materials, make_spheres, random_scene,
-                                scene_from_camera_overrides)
+from ataraxia_amd.scene import (FlatScene, SplitMix64, batch_mix_scenes, make_lights, make_materials, make_spheres,
+                                random_scene, scene_from_camera_overrides)
 
 from _scenes import mixed_scenes, oracle_accumulate
 
@@ -139,6 +139,32 @@ def test_wavefront_launch_splits_equal_plain(size, frames, launches):
     assert st_p["launches"] == frames and st_p["path_ms"] == st_p["gpu_ms"]
     assert st_w["launches"] == launches
     assert 0.0 < st_w["path_ms"] <= st_w["gpu_ms"]
+
+
+def test_mixed_workload_batch_equals_plain_kernel():
+    """The 64 scenes of the mixed workload (16..4096 spheres, 1..16 lights,
+    2..10 bounces) at 1/12 of their resolution and at most 40 spp (two
+    launches for scenes past 32 frames): each batched image equals rendering
+    that scene alone with the plain kernel, bit for bit."""
+    scenes = []
+    for s in batch_mix_scenes():
+        r = scene_from_camera_overrides(s, max(8, s.camera.width // 12), max(8, s.camera.height // 12))
+        r.spp = min(s.spp, 40)
+        scenes.append(r)
+    accs, imgs, _, st = render_batch(scenes, tuning=dict(chunk_px=128, frames_per_launch=32))
+    assert st["samples"] == sum(s.camera.width * s.camera.height * s.spp for s in scenes)
+    assert st["box_tests"] > 0
+    for i, s in enumerate(scenes):
+        with Renderer() as r:
+            r.set_kernel(ATX_KERNEL_PLAIN)
+            r.on_resize(s.camera.width, s.camera.height)
+            r.set_settings(s.settings)
+            r.set_scene(s)
+            r.render_frames(s.camera, s.spp)
+            acc = r.get_accumulation().reshape(-1, 4)
+            img = r.get_image().reshape(-1)
+        assert np.array_equal(accs[i], acc), s.name
+        assert np.array_equal(imgs[i], img), s.name
 
 
 def test_batch_argument_errors():

@@ -63,6 +63,17 @@ def test_server_staggered_arrivals_and_budget():
         assert np.array_equal(img, rimg), s.name
 
 
+def test_server_flat_loop_matches_oracle():
+    """accel = NONE: every tick runs the flat-loop kernel instance."""
+    scenes = mixed_scenes()[:4]
+    res, st = serve_all(scenes, frames_per_tick=2, accel=_lib.ATX_ACCEL_NONE)
+    assert st["completed"] == len(scenes)
+    for (img, acc), s in zip(res, scenes):
+        oacc, oimg = oracle_accumulate(s, s.spp)
+        assert np.array_equal(acc, oacc), s.name
+        assert np.array_equal(img, oimg), s.name
+
+
 def test_server_kahan_matches_oracle():
     scenes = mixed_scenes()[:3]
     res, _ = serve_all(scenes, frames_per_tick=2, kahan=True)
