@@ -79,7 +79,7 @@ class BatchTuning(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("frames_per_tick", C.c_uint32), ("chunk_px", C.c_uint32),
                 ("max_active", C.c_uint32), ("kahan", C.c_uint32), ("accel", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("tick_mpaths", C.c_uint32)]
 
 
 class ServerStats(C.Structure):

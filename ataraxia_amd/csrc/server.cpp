@@ -1,12 +1,18 @@
 // server.cpp -- continuous-batching render service on one GPU.
 //
 // Submitters copy their scene into a request and return at once.  One worker
-// thread owns the device: between launches it admits queued requests in FIFO
-// order while their device buffers fit the HBM budget (geometry, accumulation,
-// optional Kahan compensation, RGBA8 image; the head of the queue is never
-// overtaken), then renders every admitted request together with one launch of
-// the persistent wavefront kernel (frames_per_tick frames per request, each
-// request at its own frame index).  Requests that arrive while a launch runs
+// thread owns the device: between launches it admits queued requests while
+// the admitted ones fill less than tick_mpaths million paths per launch (so
+// the device stays full without every queued request sharing it) and their
+// device buffers fit the HBM budget (geometry, accumulation, optional Kahan
+// compensation, RGBA8 image, primary directions, the tick's samples).  The
+// next request is the one with the highest response ratio, (waiting time +
+// estimated render time) / estimated render time, with the render time
+// estimated from the throughput measured so far: short requests go first and
+// a long one's priority grows while it waits.  Every admitted request is
+// rendered together with the others by one launch of the persistent
+// wavefront kernel per tick (frames_per_tick frames per request, each request
+// at its own frame index).  Requests that arrive while a launch runs
 // join at the next launch.  A request that reaches its spp is copied to host
 // memory and its device buffers are released immediately, which makes room
 // for the next admissions.  Per-request timestamps (submit, admit, done) give
@@ -69,6 +75,7 @@ struct Server {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
+    double rendered = 0.0, busy_s = 0.0;   // samples rendered and GPU seconds so far (worker)
     uint32_t grid = 0, grid_flat = 0;   // persistent grids: ticks with trees, flat-loop-only ticks
     int device = 0;
 };
@@ -87,6 +94,12 @@ static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_ti
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
            npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16;
+}
+
+// Paths a request adds to the next launch.
+static uint64_t tick_paths_of(const Request& r, uint32_t frames_per_tick)
+{
+    return (uint64_t)r.cam.width * r.cam.height * std::min(frames_per_tick, r.spp - r.frames_done);
 }
 
 // Device allocations of one request (worker thread).
@@ -161,14 +174,36 @@ static void worker_main(Server* s)
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty() || !s->active.empty(); });
             if (s->stop) break;
+            const uint32_t F = s->cfg.frames_per_tick;
+            const uint64_t target =
+                s->cfg.tick_mpaths == UINT32_MAX ? UINT64_MAX : (uint64_t)s->cfg.tick_mpaths * 1000000ull;
+            uint64_t tick_paths = 0;
+            for (const auto& r : s->active) tick_paths += tick_paths_of(*r, F);
+            // samples per second: measured, or a nominal rate before the first tick
+            const double rate = s->busy_s > 0.0 ? s->rendered / s->busy_s : 4e9;
+            const double now = now_s(s);
             while (!s->queue.empty()) {
-                auto& r = s->queue.front();
+                const bool idle = s->active.empty() && newly.empty();
+                if (!idle && tick_paths >= target) break;
                 if (s->cfg.max_active && s->active.size() + newly.size() >= s->cfg.max_active) break;
+                size_t pick = 0;
+                double best = -1.0;
+                for (size_t k = 0; k < s->queue.size(); ++k) {
+                    const Request& q = *s->queue[k];
+                    const double est = std::max(1e-6, (double)q.cam.width * q.cam.height * q.spp / rate);
+                    const double ratio = (now - q.t_submit + est) / est;
+                    if (ratio > best) {
+                        best = ratio;
+                        pick = k;
+                    }
+                }
+                auto r = s->queue[pick];
                 if (s->stats.hbm_in_use + r->device_bytes > s->cfg.hbm_budget_bytes) break;
                 s->stats.hbm_in_use += r->device_bytes;
                 s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
+                tick_paths += tick_paths_of(*r, F);
                 newly.push_back(r);
-                s->queue.pop_front();
+                s->queue.erase(s->queue.begin() + (ptrdiff_t)pick);
             }
         }
         for (auto& r : newly) {
@@ -217,6 +252,10 @@ static void worker_main(Server* s)
         }
         float ms = 0.0f;
         if (ok) (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        if (ok) {
+            s->busy_s += ms * 1e-3;
+            for (const WfScene& w : table) s->rendered += (double)w.width * w.height * w.frames;
+        }
         size_t free_b = 0, total_b = 0;
         (void)hipMemGetInfo(&free_b, &total_b);
         {
@@ -268,6 +307,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     if (!s) { set_error("out of host memory"); return ATX_ERR_NOMEM; }
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = 8;
+    if (!s->cfg.tick_mpaths) s->cfg.tick_mpaths = 32;
     if (!s->cfg.chunk_px) s->cfg.chunk_px = kDefaultChunkPx;
     ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");

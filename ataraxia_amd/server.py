@@ -33,8 +33,9 @@ def scene_desc(scene: FlatScene, spp: Optional[int] = None):
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 0, frames_per_tick: int = 0, chunk_px: int = 0,
-                 max_active: int = 0, kahan: bool = False, accel: int = _lib.ATX_ACCEL_BVH):
-        cfg = _lib.ServerConfig(hbm_budget_bytes, frames_per_tick, chunk_px, max_active, 1 if kahan else 0, accel, 0)
+                 max_active: int = 0, kahan: bool = False, accel: int = _lib.ATX_ACCEL_BVH, tick_mpaths: int = 0):
+        cfg = _lib.ServerConfig(hbm_budget_bytes, frames_per_tick, chunk_px, max_active, 1 if kahan else 0, accel,
+                                tick_mpaths)
         h = C.c_void_p()
         check(lib.atx_server_create(C.byref(h), C.byref(cfg)))
         self._h = h

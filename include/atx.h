@@ -203,13 +203,16 @@ int atx_batch_get_accumulation(atx_batch* b, uint32_t scene_id, float* rgba, uin
 int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 
 /* ---- Server: continuous-batching render service ----------------------------
- * No reference counterpart.  Requests (a scene + spp) are admitted in FIFO
- * order while their device buffers fit the HBM budget; every admitted request
- * stays device-resident until it completes.  A worker thread renders all
- * admitted requests together, frames_per_tick frames per request per launch of
- * the persistent kernel, and admits newly arrived requests between launches.
- * A finished request's image and accumulation are copied to host memory and
- * its device buffers are released at once.                                  */
+ * No reference counterpart.  A worker thread renders all admitted requests
+ * (a scene + spp) together, frames_per_tick frames per request per launch of
+ * the persistent kernel, and admits queued requests between launches while
+ * the admitted ones fill less than tick_mpaths million paths per launch and
+ * their device buffers fit the HBM budget.  Among queued requests the one
+ * with the highest response ratio (waiting time + estimated render time) /
+ * estimated render time goes first: short requests overtake long ones, and a
+ * long one's priority grows while it waits, so none starves.  Every admitted
+ * request stays device-resident until it completes; its image and
+ * accumulation are then copied to host memory and its buffers released.    */
 #define ATX_REQ_QUEUED 0
 #define ATX_REQ_RUNNING 1
 #define ATX_REQ_DONE 2
@@ -218,11 +221,12 @@ int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 typedef struct atx_server_config {
     uint64_t hbm_budget_bytes;  /* device bytes for request buffers (0 = half of free HBM) */
     uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 8)          */
-    uint32_t chunk_px;          /* pixels per work chunk (0 = 128)                          */
+    uint32_t chunk_px;          /* pixels per work chunk (0 = 256)                          */
     uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
     uint32_t kahan;             /* 1 = compensated accumulation                             */
     uint32_t accel;             /* ATX_ACCEL_BVH (0) or ATX_ACCEL_NONE                      */
-    uint32_t reserved;
+    uint32_t tick_mpaths;       /* admit until a launch holds this many million paths
+                                   (0 = 32; UINT32_MAX = admit whatever fits the budget)    */
 } atx_server_config;
 
 typedef struct atx_server_stats {

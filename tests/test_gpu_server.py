@@ -10,8 +10,9 @@ import numpy as np
 import pytest
 
 from _scenes import mixed_scenes, oracle_accumulate
-from ataraxia_amd import AtxError, Server
+from ataraxia_amd import AtxError, Server, plain_1080p_scene
 from ataraxia_amd import _lib
+from ataraxia_amd.scene import scene_from_camera_overrides
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +73,27 @@ def test_server_flat_loop_matches_oracle():
         oacc, oimg = oracle_accumulate(s, s.spp)
         assert np.array_equal(acc, oacc), s.name
         assert np.array_equal(img, oimg), s.name
+
+
+def test_server_runs_short_requests_first():
+    """One resident request at a time: while a long first request renders,
+    a long and three short requests queue; the short ones (highest response
+    ratio) finish before the long one submitted ahead of them."""
+    blocker = plain_1080p_scene()
+    blocker.spp = 256
+    long_req = scene_from_camera_overrides(plain_1080p_scene(), 640, 360)
+    long_req.spp = 64
+    short = [mixed_scenes()[k] for k in (0, 1, 3)]
+    with Server(max_active=1) as srv:
+        ids = [srv.submit(blocker), srv.submit(long_req)] + [srv.submit(s) for s in short]
+        for rid in ids:
+            srv.wait(rid, 60.0)
+        done = [srv.info(rid)["t_done"] for rid in ids]
+        results = [srv.result(rid, release=True) for rid in ids[2:]]
+    assert max(done[2:]) < done[1], done
+    for (img, acc), s in zip(results, short):
+        oacc, _ = oracle_accumulate(s, s.spp)
+        assert np.array_equal(acc.reshape(-1, 4), oacc), s.name
 
 
 def test_server_kahan_matches_oracle():
