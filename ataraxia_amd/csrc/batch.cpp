@@ -158,8 +158,11 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
         WfPass lp{};
-        layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], b->chunk_px, b->d_samples.as<float4>(),
-                      &lp);
+        if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], b->chunk_px,
+                           b->d_samples.as<float4>(), &lp, b->flat_only)) {
+            set_error("batch: a launch of %u frames holds 2^31 or more paths; lower frames_per_launch", F);
+            return ATX_ERR_INVALID;
+        }
         b->launch_chunks.push_back(lp.total_chunks);
         b->launch_px.push_back(lp.total_px);
     }

@@ -131,7 +131,10 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
         for (WfScene& e : table) {
             WfPass lp{};
-            layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp);
+            if (!layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp, e.bvh.n_nodes == 0)) {
+                set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
+                return ATX_ERR_INVALID;
+            }
         }
         ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
         for (WfScene& e : table) e.dirs = r->wf_dirs.as<float4>();

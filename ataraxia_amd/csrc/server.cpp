@@ -96,6 +96,9 @@ static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_ti
            npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16;
 }
 
+// A launch's paths must stay below 2^31 (the kernel's 32-bit job indices).
+constexpr uint64_t kMaxTickPaths = 1ull << 31;
+
 // Paths a request adds to the next launch.
 static uint64_t tick_paths_of(const Request& r, uint32_t frames_per_tick)
 {
@@ -199,6 +202,7 @@ static void worker_main(Server* s)
                 }
                 auto r = s->queue[pick];
                 if (s->stats.hbm_in_use + r->device_bytes > s->cfg.hbm_budget_bytes) break;
+                if (tick_paths + tick_paths_of(*r, F) >= kMaxTickPaths) break;   // 32-bit job indices
                 s->stats.hbm_in_use += r->device_bytes;
                 s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
                 tick_paths += tick_paths_of(*r, F);
@@ -235,7 +239,8 @@ static void worker_main(Server* s)
         WfPass wp{};
         bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
                                      sizeof(float4)) == hipSuccess;
-        if (ok) layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
+        const bool flat = wf_flat_only(table.data(), table.size());
+        if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp, flat);
         ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
                   hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene),
                                  hipMemcpyHostToDevice, s->stream) == hipSuccess &&
@@ -246,7 +251,6 @@ static void worker_main(Server* s)
             wp.n_scenes = (uint32_t)table.size();
             wp.queue = s->queue_ctr.as<unsigned int>();
             wp.counters = s->counters.as<unsigned long long>();
-            const bool flat = wf_flat_only(table.data(), table.size());
             ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
                  hipEventRecord(s->ev1, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
         }
@@ -370,6 +374,9 @@ int atx_server_submit(atx_server* h, const atx_scene_desc* d, uint64_t* request_
     ATX_CHECK_ARG((uint64_t)d->camera.width * d->camera.height < (1ull << 31), "viewport too large");
     ATX_CHECK_ARG(d->settings.max_bounces >= 0, "max_bounces must be >= 0");
     ATX_CHECK_ARG(d->spp >= 1, "spp must be >= 1");
+    ATX_CHECK_ARG((uint64_t)d->camera.width * d->camera.height * std::min(d->spp, s->cfg.frames_per_tick) <
+                      kMaxTickPaths,
+                  "one tick of this request holds 2^31 or more paths; lower frames_per_tick");
     ATX_CHECK_ARG(d->num_spheres == 0 || d->spheres, "spheres is NULL");
     ATX_CHECK_ARG(d->num_materials == 0 || d->materials, "materials is NULL");
     ATX_CHECK_ARG(d->num_lights == 0 || d->lights, "lights is NULL");

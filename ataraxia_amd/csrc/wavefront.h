@@ -4,11 +4,13 @@
 //
 // One launch renders frames [frame0, frame0 + frames) of every scene in its
 // table in two kernels.  The path kernel treats every (frame, pixel) path as
-// an independent job: waves take blocks of `chunk_px` consecutive pixels of
-// one frame of one scene from a global counter (chunk ids are contiguous per
-// scene and frame-major within it: scene s owns [chunk_begin, chunk_begin +
-// n_chunks), n_chunks = frames * ceil(pixels / chunk_px)) and hand the
-// block's paths to their lanes one by one; a lane whose path ends stores its
+// an independent job; a scene's jobs are numbered pixel-major (job j: pixel
+// j / frames, frame frame0 + j % frames) in launches with trees and
+// frame-major in flat-loop-only ones.  Waves take blocks of `chunk_px`
+// consecutive jobs of one scene from a global counter (scene s owns chunk
+// ids [chunk_begin, chunk_begin + n_chunks), n_chunks = ceil(pixels * frames
+// / chunk_px)) and hand the block's paths to their lanes one by one; a lane
+// whose path ends stores its
 // sample in the launch's sample buffer and takes the next job, so every
 // closest-hit loop runs with (nearly) all 64 lanes busy and no wave waits for
 // another's work.  The accumulate kernel then adds each pixel's samples to
@@ -41,7 +43,7 @@ struct WfScene {
     uint32_t frame0;       // first frame index rendered by this launch (>= 1)
     uint32_t frames;       // frames rendered by this launch (>= 1)
     uint32_t chunk_begin;  // first chunk id of this scene in the launch
-    uint32_t n_chunks;     // frames * ceil(width * height / chunk_px)
+    uint32_t n_chunks;     // ceil(width * height * frames / chunk_px)
     uint32_t kahan;        // 1: compensated accumulation (comp must be valid)
     uint32_t dbg_bounces;  // stride of the debug arrays
     uint32_t px_begin;     // first pixel of this scene in the launch's pixel range
@@ -63,7 +65,7 @@ struct WfPass {
     const WfScene* scenes;
     uint32_t n_scenes;
     uint32_t total_chunks; // chunk ids [0, total_chunks)
-    uint32_t chunk_px;     // pixels per chunk (multiple of 64, <= kMaxChunkPx)
+    uint32_t chunk_px;     // jobs per chunk (multiple of 64, <= kMaxChunkPx)
     uint32_t total_px;     // pixels of all scenes of the launch
     unsigned int* queue;   // this launch's chunk counter (zero before launch)
     unsigned long long* counters;

@@ -41,10 +41,6 @@ inline void fill_wf_scene(const DevScene& g, const atx_settings& st, const atx_c
     out->bvh = g.accel(accel);
 }
 
-inline uint32_t chunks_for(uint32_t pixels, uint32_t chunk_items)
-{
-    return (pixels + chunk_items - 1) / chunk_items;
-}
 
 // Sample-buffer floats4 one launch of these table entries needs.
 inline size_t launch_samples(const WfScene* tab, size_t n)
@@ -57,24 +53,36 @@ inline size_t launch_samples(const WfScene* tab, size_t n)
 
 // Gives the launch's table entries their chunk ids, pixel offsets and slices
 // of the sample buffer (in table order) and fills the launch's totals.
-inline void layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p)
+// Launches with trees number a scene's jobs pixel-major in chunks of chunk_px
+// jobs (ceil(pixels * frames / chunk_px) chunks); flat-loop-only launches
+// (flat_only) use chunks of chunk_px pixels of one frame (frames *
+// ceil(pixels / chunk_px)).  False when a count would not fit the kernel's
+// 32-bit indices (a scene's jobs, or the launch's pixels or chunks, reach
+// 2^31).
+inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p, bool flat_only)
 {
-    uint32_t chunks = 0, px = 0;
+    const uint64_t kLimit = 1ull << 31;
+    uint64_t chunks = 0, px = 0;
     size_t off = 0;
     for (size_t k = 0; k < n; ++k) {
         WfScene& w = tab[k];
-        const uint32_t npx = w.width * w.height;
-        w.chunk_begin = chunks;
-        w.n_chunks = chunks_for(npx, chunk_px) * w.frames;
-        w.px_begin = px;
+        const uint64_t npx = (uint64_t)w.width * w.height;
+        const uint64_t jobs = npx * w.frames;
+        const uint64_t nch = flat_only ? ((npx + chunk_px - 1) / chunk_px) * w.frames : (jobs + chunk_px - 1) / chunk_px;
+        if (jobs >= kLimit || chunks + nch >= kLimit || px + npx >= kLimit)
+            return false;
+        w.chunk_begin = (uint32_t)chunks;
+        w.n_chunks = (uint32_t)nch;
+        w.px_begin = (uint32_t)px;
         w.samples = samples + off;
-        chunks += w.n_chunks;
+        chunks += nch;
         px += npx;
-        off += (size_t)npx * w.frames;
+        off += (size_t)jobs;
     }
-    p->total_chunks = chunks;
-    p->total_px = px;
+    p->total_chunks = (uint32_t)chunks;
+    p->total_px = (uint32_t)px;
     p->chunk_px = chunk_px;
+    return true;
 }
 
 inline uint32_t device_cus()

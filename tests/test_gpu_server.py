@@ -85,7 +85,11 @@ def test_server_runs_short_requests_first():
     long_req.spp = 64
     short = [mixed_scenes()[k] for k in (0, 1, 3)]
     with Server(max_active=1) as srv:
-        ids = [srv.submit(blocker), srv.submit(long_req)] + [srv.submit(s) for s in short]
+        ids = [srv.submit(blocker)]
+        deadline = time.monotonic() + 30.0
+        while srv.info(ids[0])["state"] == _lib.ATX_REQ_QUEUED and time.monotonic() < deadline:
+            time.sleep(0.0005)
+        ids += [srv.submit(long_req)] + [srv.submit(s) for s in short]
         for rid in ids:
             srv.wait(rid, 60.0)
         done = [srv.info(rid)["t_done"] for rid in ids]
