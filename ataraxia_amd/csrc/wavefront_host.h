@@ -109,8 +109,8 @@ inline bool wf_flat_only(const WfScene* tab, size_t n)
     return true;
 }
 
-constexpr uint32_t kDefaultChunkPx = 256;
-constexpr uint32_t kDefaultFramesPerLaunch = 32;
+constexpr uint32_t kDefaultChunkPx = 512;
+constexpr uint32_t kDefaultFramesPerLaunch = 64;
 // Upper bound of one launch's sample buffer (16 B per pixel and frame):
 // renderers shorten their launches, batches split them by scene groups.
 constexpr size_t kSampleBufferBytes = size_t(2) << 30;

@@ -5,8 +5,8 @@
 # then summarise with: python ataraxia_amd/profiling/summarize.py <OUT>
 #   1. kernel trace + stats of bench.py (STEPS frames of the 1080p workload)
 #   2. FETCH_SIZE, then WRITE_SIZE, each in its own pass, over one launch:
-#      PMC_STEPS frames (plain: one launch per frame; wavefront: 32 frames
-#      per launch, so PMC_STEPS=32 is exactly one launch)
+#      PMC_STEPS frames (plain: one launch per frame; wavefront: 64 frames
+#      per launch, so PMC_STEPS=64 is exactly one launch)
 #   3. the same two counters on copy_probe (256 MiB copy) for calibration
 #   4. SQ counters (waves, VALU instructions and lane utilisation, waits, loads)
 # Every pass has its own time limit; the script stops at the first failure.
@@ -15,8 +15,8 @@ REPO=$(pwd)
 KERNEL=${KERNEL:-wavefront}
 ACCEL=${ACCEL:-bvh}
 OUT=${OUT:-$REPO/profile_out/prof_${KERNEL}_$ACCEL}
-STEPS=${STEPS:-64}
-if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-32}; fi
+STEPS=${STEPS:-128}
+if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-64}; fi
 BENCH=("$REPO/bench.py" --kernel "$KERNEL" --accel "$ACCEL" --no-serve --no-cpu-baseline)
 mkdir -p "$OUT"
 export TMPDIR=/tmp

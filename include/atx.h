@@ -170,10 +170,10 @@ typedef struct atx_scene_desc {
 } atx_scene_desc;
 
 typedef struct atx_batch_tuning {
-    uint32_t chunk_px;          /* pixels per work chunk: multiple of 64, <= 1024 (0 = 128) */
+    uint32_t chunk_px;          /* paths per work chunk: multiple of 64, <= 1024 (0 = 512)  */
     uint32_t workgroups;        /* persistent grid, 256-lane workgroups (0 = fill device) */
     uint32_t use_graph;         /* capture all launches in a HIP graph and replay it       */
-    uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 16)           */
+    uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 64)           */
 } atx_batch_tuning;
 
 typedef struct atx_batch atx_batch;
@@ -221,7 +221,7 @@ int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 typedef struct atx_server_config {
     uint64_t hbm_budget_bytes;  /* device bytes for request buffers (0 = half of free HBM) */
     uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 8)          */
-    uint32_t chunk_px;          /* pixels per work chunk (0 = 256)                          */
+    uint32_t chunk_px;          /* paths per work chunk (0 = 512)                           */
     uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
     uint32_t kahan;             /* 1 = compensated accumulation                             */
     uint32_t accel;             /* ATX_ACCEL_BVH (0) or ATX_ACCEL_NONE                      */

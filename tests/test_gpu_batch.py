@@ -112,10 +112,10 @@ def test_wavefront_equals_plain_on_full_1080p():
     assert out[ATX_KERNEL_PLAIN][2] == out[ATX_KERNEL_WAVEFRONT][2]
 
 
-@pytest.mark.parametrize("size,frames,launches", [((48, 30), 70, 3), ((4096, 4096), 20, 3)],
-                         ids=["32-frame-launches", "2GB-sample-cap"])
+@pytest.mark.parametrize("size,frames,launches", [((48, 30), 150, 3), ((4096, 4096), 20, 3)],
+                         ids=["64-frame-launches", "2GB-sample-cap"])
 def test_wavefront_launch_splits_equal_plain(size, frames, launches):
-    """70 frames split into 32 + 32 + 6 frames per launch, and a 4096x4096
+    """150 frames split into 64 + 64 + 22 frames per launch, and a 4096x4096
     image whose 2 GB sample buffer holds 8 frames (8 + 8 + 4): both must
     match the plain kernel's frame-by-frame accumulation bit for bit."""
     w, h = size
@@ -191,7 +191,7 @@ def test_autotune_picks_valid_tuning_and_caches_it(tmp_path):
     with Batch() as b:
         ids = [b.add_scene(s) for s in scenes]
         t1 = b.autotune(str(cache))
-        assert t1["chunk_px"] in (64, 128, 256) and t1["frames_per_launch"] in (16, 32, 64)
+        assert t1["chunk_px"] in (128, 256, 512, 1024) and t1["frames_per_launch"] in (16, 32, 64)
         assert t1["workgroups"] > 0
         b.render()
         accs = [b.get_accumulation(i).reshape(-1, 4).copy() for i in ids]
