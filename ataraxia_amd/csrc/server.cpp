@@ -310,7 +310,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     std::unique_ptr<Server> s(new (std::nothrow) Server());
     if (!s) { set_error("out of host memory"); return ATX_ERR_NOMEM; }
     if (cfg) s->cfg = *cfg;
-    if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = 8;
+    if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = kDefaultFramesPerLaunch;
     if (!s->cfg.tick_mpaths) s->cfg.tick_mpaths = 32;
     if (!s->cfg.chunk_px) s->cfg.chunk_px = kDefaultChunkPx;
     ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be a multiple of 64, at most 1024");

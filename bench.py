@@ -61,7 +61,10 @@ def parse():
     ap.add_argument("--spp-scale", type=float, default=1.0, help="batch_mix: scale every scene's spp")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
-    ap.add_argument("--frames-per-tick", type=int, default=8, help="serve: frames per request per launch")
+    ap.add_argument("--frames-per-tick", type=int, default=0,
+                    help="serve: frames per request per launch (0: the server's default)")
+    ap.add_argument("--tick-mpaths", type=int, default=0,
+                    help="serve: million paths per launch before admitting more (0: the server's default)")
     ap.add_argument("--no-serve", action="store_true", help="plain_1080p: skip the short serving run")
     return ap.parse_args()
 
@@ -269,7 +272,7 @@ def bench_serve(args, rate=None, requests=None):
         t += -math.log(1.0 - arr.uniform()) / rate
         arrivals.append(t)
     from ataraxia_amd import ATX_ACCEL_BVH, ATX_ACCEL_NONE
-    srv = Server(frames_per_tick=args.frames_per_tick,
+    srv = Server(frames_per_tick=args.frames_per_tick, tick_mpaths=args.tick_mpaths,
                  accel=ATX_ACCEL_NONE if args.accel == "none" else ATX_ACCEL_BVH)
     t0 = time.perf_counter()
     ids = []
@@ -291,7 +294,8 @@ def bench_serve(args, rate=None, requests=None):
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "hbm_high_water_bytes": st["hbm_high_water"],
             "device_used_high_water_bytes": st["device_used_high_water"], "ticks": st["ticks"],
-            "gpu_busy_ms": st["gpu_busy_ms"], "frames_per_tick": args.frames_per_tick,
+            "gpu_busy_ms": st["gpu_busy_ms"], "frames_per_tick": args.frames_per_tick or "default",
+            "tick_mpaths": args.tick_mpaths or "default",
             "requests_spec": "resolution from {512x512, 1280x720, 1920x1080}, spheres logU[4,1024] + ground, "
                              "spp logU[16,1024], 8 bounces, 1..8 lights"}
 

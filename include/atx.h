@@ -220,7 +220,7 @@ int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 
 typedef struct atx_server_config {
     uint64_t hbm_budget_bytes;  /* device bytes for request buffers (0 = half of free HBM) */
-    uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 8)          */
+    uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 64)         */
     uint32_t chunk_px;          /* paths per work chunk (0 = 512)                           */
     uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
     uint32_t kahan;             /* 1 = compensated accumulation                             */
