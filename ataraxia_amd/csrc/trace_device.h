@@ -109,6 +109,10 @@ __device__ __forceinline__ bool filter_sphere(float4 s, int32_t id, f3 o, f3 d, 
     const float b = (ocx * d.x + ocy * d.y) + ocz * d.z;
     const float c = ((ocx * ocx + ocy * ocy) + ocz * ocz) - s.w;
     const float disc = b * b - a * c;
+    // No lane's line meets the sphere (disc < 0 or NaN): the bounds below
+    // would be NaN and change nothing, so the wave skips them.
+    if (!__any(disc >= 0.0f))
+        return false;
     const float sq = __builtin_amdgcn_sqrtf(disc);  // NaN when disc < 0
     const float nn = -b - sq;
     // The floor keeps the interval sound where v_sqrt_f32 is not trusted
