@@ -159,7 +159,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
         WfPass lp{};
         if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], b->chunk_px,
-                           b->d_samples.as<float4>(), &lp, b->flat_only)) {
+                           b->d_samples.as<float4>(), &lp)) {
             set_error("batch: a launch of %u frames holds 2^31 or more paths; lower frames_per_launch", F);
             return ATX_ERR_INVALID;
         }

@@ -19,14 +19,13 @@
 //
 // Work distribution: a wave takes blocks of `chunk_px` consecutive jobs of
 // one scene from the launch's global counter and hands them to its lanes as
-// they fall idle.  Launches that traverse trees number a scene's jobs
-// pixel-major, the launch's frames of one pixel next to each other (job j =
-// pixel j / frames, frame frame0 + j % frames), so a wave traces a few
-// neighbouring pixels in many frames at once: their primary rays are the
-// same ray (there is no jitter) and their first hits and shadow rays mostly
-// shared, which keeps the lanes of a wave on the same tree nodes.  The flat
-// loop has no such divergence to remove; its launches hand out blocks of
-// chunk_px pixels of one frame (frame-major), which measured faster for it.  Samples are stored frame-major ([frame][pixel]), so that
+// they fall idle.  A scene's jobs are numbered pixel-major, the launch's
+// frames of one pixel next to each other (job j = pixel j / frames, frame
+// frame0 + j % frames), so a wave traces a few neighbouring pixels in many
+// frames at once: their primary rays are the same ray (there is no jitter)
+// and their first hits and shadow rays mostly shared, which keeps the lanes
+// of a wave on the same tree nodes and makes whole waves miss the same
+// spheres in the flat loop.  Samples are stored frame-major ([frame][pixel]), so that
 // k_accumulate reads them coalesced; a wave writes all frames of its few
 // pixels within a short time, and L2 merges those stores into whole lines.  Spheres and the scene table
 // are read with wave-uniform addresses through the constant address space,
@@ -96,7 +95,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long* ctr, uint32_t
     tests = boxes = 0u;
 }
 
-template <uint32_t kWaves, bool kPixelMajor>
+template <uint32_t kWaves>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_render_wavefront(WfPass a)
 {
     const uint32_t lane = threadIdx.x & 63u;
@@ -109,9 +108,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     CScene* S = nullptr;
     int32_t pend = -1;              // a block of another scene, waiting for the lanes to drain
     bool qdone = false;
-    // jobs left in the wave's block: [blk0 + next, blk0 + end) of scene S
-    // (pixel-major: job numbers; frame-major: pixels of frame blk_frame)
-    uint32_t blk_frame = 0, blk0 = 0, blk_next = 0, blk_end = 0;
+    uint32_t blk0 = 0, blk_next = 0, blk_end = 0;   // jobs left: [blk0 + next, blk0 + end) of scene S
 
     // ---- per-lane path state ----
     uint32_t state = kIdle;
@@ -140,15 +137,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
                 const uint32_t nidle = (uint32_t)__popcll(idle);
                 if (state == kIdle && rank < avail) {
-                    if constexpr (kPixelMajor) {
-                        // job j: pixel j / frames, frame frame0 + j % frames
-                        const uint32_t j = blk0 + blk_next + rank;
-                        px = j / S->frames;
-                        frame = S->frame0 + (j - px * S->frames);
-                    } else {
-                        px = blk0 + blk_next + rank;
-                        frame = blk_frame;
-                    }
+                    // job j: pixel j / frames, frame frame0 + j % frames
+                    const uint32_t j = blk0 + blk_next + rank;
+                    px = j / S->frames;
+                    frame = S->frame0 + (j - px * S->frames);
                     ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
                     if (S->dirs) {
                         const float4 dv = S->dirs[px];
@@ -208,20 +200,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 S = &scenes[s];
             }
             pend = -1;
-            const uint32_t rel = (uint32_t)c - S->chunk_begin;
-            if constexpr (kPixelMajor) {
-                const uint32_t jobs = S->width * S->height * S->frames;
-                blk0 = rel * P;
-                blk_end = blk0 >= jobs ? 0u : (jobs - blk0 < P ? jobs - blk0 : P);
-            } else {
-                // chunk: P pixels of one frame (ceil(pixels / P) chunks per frame)
-                const uint32_t npx = S->width * S->height;
-                const uint32_t per_frame = (npx + P - 1u) / P;
-                const uint32_t fo = rel / per_frame;
-                blk_frame = S->frame0 + fo;
-                blk0 = (rel - fo * per_frame) * P;
-                blk_end = fo >= S->frames ? 0u : (npx - blk0 < P ? npx - blk0 : P);
-            }
+            const uint32_t jobs = S->width * S->height * S->frames;
+            blk0 = ((uint32_t)c - S->chunk_begin) * P;
+            blk_end = blk0 >= jobs ? 0u : (jobs - blk0 < P ? jobs - blk0 : P);
             blk_next = 0;
         }
         if (!__any(state != kIdle) && qdone && pend < 0)
@@ -419,9 +400,9 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
     if (l.flat_only)
-        hipLaunchKernelGGL((k_render_wavefront<kWfWavesFlat, false>), dim3(l.workgroups), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_render_wavefront<kWfWavesFlat>, dim3(l.workgroups), dim3(256), 0, stream, p);
     else
-        hipLaunchKernelGGL((k_render_wavefront<kWfWavesTree, true>), dim3(l.workgroups), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_render_wavefront<kWfWavesTree>, dim3(l.workgroups), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && paths_done)
         e = hipEventRecord(paths_done, stream);
@@ -460,8 +441,8 @@ int wavefront_max_blocks_per_cu(bool flat_only)
 {
     int blocks = 0;
     const hipError_t e =
-        flat_only ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesFlat, false>, 256, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesTree, true>, 256, 0);
+        flat_only ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesFlat>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesTree>, 256, 0);
     if (e != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;

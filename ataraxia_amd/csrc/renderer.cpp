@@ -131,7 +131,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
         for (WfScene& e : table) {
             WfPass lp{};
-            if (!layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp, e.bvh.n_nodes == 0)) {
+            if (!layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp)) {
                 set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
                 return ATX_ERR_INVALID;
             }

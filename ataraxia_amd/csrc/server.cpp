@@ -239,8 +239,7 @@ static void worker_main(Server* s)
         WfPass wp{};
         bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
                                      sizeof(float4)) == hipSuccess;
-        const bool flat = wf_flat_only(table.data(), table.size());
-        if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp, flat);
+        if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
         ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
                   hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene),
                                  hipMemcpyHostToDevice, s->stream) == hipSuccess &&
@@ -251,6 +250,7 @@ static void worker_main(Server* s)
             wp.n_scenes = (uint32_t)table.size();
             wp.queue = s->queue_ctr.as<unsigned int>();
             wp.counters = s->counters.as<unsigned long long>();
+            const bool flat = wf_flat_only(table.data(), table.size());
             ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
                  hipEventRecord(s->ev1, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
         }

@@ -5,8 +5,7 @@
 // One launch renders frames [frame0, frame0 + frames) of every scene in its
 // table in two kernels.  The path kernel treats every (frame, pixel) path as
 // an independent job; a scene's jobs are numbered pixel-major (job j: pixel
-// j / frames, frame frame0 + j % frames) in launches with trees and
-// frame-major in flat-loop-only ones.  Waves take blocks of `chunk_px`
+// j / frames, frame frame0 + j % frames).  Waves take blocks of `chunk_px`
 // consecutive jobs of one scene from a global counter (scene s owns chunk
 // ids [chunk_begin, chunk_begin + n_chunks), n_chunks = ceil(pixels * frames
 // / chunk_px)) and hand the block's paths to their lanes one by one; a lane

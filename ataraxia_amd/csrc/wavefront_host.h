@@ -51,15 +51,12 @@ inline size_t launch_samples(const WfScene* tab, size_t n)
     return s;
 }
 
-// Gives the launch's table entries their chunk ids, pixel offsets and slices
-// of the sample buffer (in table order) and fills the launch's totals.
-// Launches with trees number a scene's jobs pixel-major in chunks of chunk_px
-// jobs (ceil(pixels * frames / chunk_px) chunks); flat-loop-only launches
-// (flat_only) use chunks of chunk_px pixels of one frame (frames *
-// ceil(pixels / chunk_px)).  False when a count would not fit the kernel's
-// 32-bit indices (a scene's jobs, or the launch's pixels or chunks, reach
-// 2^31).
-inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p, bool flat_only)
+// Gives the launch's table entries their chunk ids (ceil(pixels * frames /
+// chunk_px) chunks of chunk_px jobs each), pixel offsets and slices of the
+// sample buffer (in table order) and fills the launch's totals.  False when
+// a count would not fit the kernel's 32-bit indices (a scene's jobs, or the
+// launch's pixels or chunks, reach 2^31).
+inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p)
 {
     const uint64_t kLimit = 1ull << 31;
     uint64_t chunks = 0, px = 0;
@@ -68,7 +65,7 @@ inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* sam
         WfScene& w = tab[k];
         const uint64_t npx = (uint64_t)w.width * w.height;
         const uint64_t jobs = npx * w.frames;
-        const uint64_t nch = flat_only ? ((npx + chunk_px - 1) / chunk_px) * w.frames : (jobs + chunk_px - 1) / chunk_px;
+        const uint64_t nch = (jobs + chunk_px - 1) / chunk_px;
         if (jobs >= kLimit || chunks + nch >= kLimit || px + npx >= kLimit)
             return false;
         w.chunk_begin = (uint32_t)chunks;
