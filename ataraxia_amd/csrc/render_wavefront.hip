@@ -34,8 +34,8 @@
 // drains its lanes before it switches.
 //
 // Registers: the kernel is compiled twice, for 6 and for 7 resident waves
-// per SIMD (amdgpu_waves_per_eu: at most 80 and 72 VGPRs, 2 and 16 spilled
-// to scratch).  More waves hide more latency than the spills cost: launches that
+// per SIMD (amdgpu_waves_per_eu: at most 80 and 72 VGPRs; the second
+// spills a few to scratch).  More waves hide more latency than the spills cost: launches that
 // traverse trees take the 6-wave instance, launches whose scenes all use the
 // flat loop the 7-wave one (the natural allocation, 93 VGPRs, allows 5).
 #include "device_types.h"
