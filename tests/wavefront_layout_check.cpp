@@ -1,0 +1,70 @@
+// Host-side checks of the persistent kernel's launch layout
+// (ataraxia_amd/csrc/wavefront_host.h): chunk ids, pixel and sample offsets,
+// launch totals, the 32-bit guard, and the flat-loop test of a table.
+// Built host-only (hipcc --offload-host-only) by tests/test_wavefront_host.py.
+#include <cstdio>
+#include <vector>
+
+#include "wavefront_host.h"
+
+static int failures = 0;
+#define CHECK(c)                                                          \
+    do {                                                                  \
+        if (!(c)) {                                                       \
+            std::fprintf(stderr, "FAIL line %d: %s\n", __LINE__, #c);     \
+            ++failures;                                                   \
+        }                                                                 \
+    } while (0)
+
+static atx::WfScene scene(uint32_t w, uint32_t h, uint32_t frames, uint32_t nodes)
+{
+    atx::WfScene s{};
+    s.width = w;
+    s.height = h;
+    s.frames = frames;
+    s.bvh.n_nodes = nodes;
+    return s;
+}
+
+int main()
+{
+    using namespace atx;
+    float4* base = reinterpret_cast<float4*>(uintptr_t(1) << 20);
+
+    // two scenes: 100x50 px x 3 frames (15000 jobs), 7x3 px x 64 frames (1344 jobs)
+    std::vector<WfScene> t = {scene(100, 50, 3, 9), scene(7, 3, 64, 0)};
+    WfPass p{};
+    CHECK(layout_launch(t.data(), t.size(), 128, base, &p));
+    CHECK(t[0].chunk_begin == 0 && t[0].n_chunks == 118);      // ceil(15000 / 128)
+    CHECK(t[1].chunk_begin == 118 && t[1].n_chunks == 11);     // ceil(1344 / 128)
+    CHECK(p.total_chunks == 129 && p.chunk_px == 128);
+    CHECK(t[0].px_begin == 0 && t[1].px_begin == 5000 && p.total_px == 5021);
+    CHECK(t[0].samples == base && t[1].samples == base + 15000);
+    CHECK(launch_samples(t.data(), t.size()) == 15000 + 1344);
+    CHECK(!wf_flat_only(t.data(), t.size()) && wf_flat_only(t.data() + 1, 1));
+
+    // every job of a scene belongs to exactly one chunk: jobs [c*P, min((c+1)*P, jobs))
+    for (const WfScene& s : t) {
+        const uint64_t jobs = (uint64_t)s.width * s.height * s.frames;
+        CHECK((uint64_t)s.n_chunks * 128 >= jobs && (uint64_t)(s.n_chunks - 1) * 128 < jobs);
+    }
+
+    // a scene of 2^31 or more jobs is refused (the kernel's job indices are 32-bit)
+    std::vector<WfScene> big = {scene(32768, 16384, 4, 1)};   // 2^31 jobs
+    CHECK(!layout_launch(big.data(), big.size(), 512, base, &p));
+    big[0].frames = 3;                                          // 1.6e9 jobs: fits
+    CHECK(layout_launch(big.data(), big.size(), 512, base, &p));
+    CHECK(big[0].n_chunks == (uint32_t)((3ull * 32768 * 16384 + 511) / 512));
+
+    // the launch's pixel total is guarded too
+    std::vector<WfScene> many(3, scene(32768, 32767, 1, 0));   // 3 x ~1.07e9 pixels
+    CHECK(!layout_launch(many.data(), many.size(), 512, base, &p));
+
+    // a single 1x1 scene of one frame: one chunk
+    std::vector<WfScene> one = {scene(1, 1, 1, 0)};
+    CHECK(layout_launch(one.data(), one.size(), 64, base, &p));
+    CHECK(one[0].n_chunks == 1 && p.total_chunks == 1 && p.total_px == 1);
+
+    if (failures == 0) std::printf("wavefront layout: all checks passed\n");
+    return failures == 0 ? 0 : 1;
+}
