@@ -141,6 +141,26 @@ def test_wavefront_launch_splits_equal_plain(size, frames, launches):
     assert 0.0 < st_w["path_ms"] <= st_w["gpu_ms"]
 
 
+def test_wavefront_accumulation_off_equals_plain():
+    """Accumulation off: every frame is frame 1 (one frame per launch); after
+    five frames both kernels hold that frame's samples, bit for bit."""
+    sc = scene_from_camera_overrides(plain_1080p_scene(), 120, 70)
+    sc.settings = Settings(False, True, 5)
+    out = {}
+    for kernel in (ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT):
+        with Renderer() as r:
+            r.set_kernel(kernel)
+            r.on_resize(120, 70)
+            r.set_settings(sc.settings)
+            r.set_scene(sc)
+            r.render_frames(sc.camera, 5)
+            assert r.frame_index == 1
+            out[kernel] = (r.get_accumulation().copy(), r.get_image().copy(), r.stats())
+    assert np.array_equal(out[ATX_KERNEL_PLAIN][0], out[ATX_KERNEL_WAVEFRONT][0])
+    assert np.array_equal(out[ATX_KERNEL_PLAIN][1], out[ATX_KERNEL_WAVEFRONT][1])
+    assert out[ATX_KERNEL_WAVEFRONT][2]["launches"] == 5
+
+
 def test_mixed_workload_batch_equals_plain_kernel():
     """The 64 scenes of the mixed workload (16..4096 spheres, 1..16 lights,
     2..10 bounces) at 1/12 of their resolution and at most 40 spp (two
