@@ -200,28 +200,33 @@ ATX_HD f3 to_world(f3 N, float hx, float hy, float hz)
     return add(add(scale(T, hx), scale(B, hy)), scale(N, hz));
 }
 
-ATX_HD f3 sample_cosine(f3 N, uint32_t& seed)
+// The next direction of a path (Renderer.cu:381-384): the GGX half-vector
+// when the material is metallic (BRDF.cu:95-116: cos theta =
+// sqrt((1 - u1) / (1 + (a^2 - 1) u1)), a = roughness^2, sin theta =
+// sqrt(1 - cos^2 theta)), else a cosine-weighted direction (BRDF.cu:72-93:
+// radius sqrt(u1), height sqrt(1 - u1)).  Both take u1 and u2 from the stream
+// in that order and place (radius cos phi, radius sin phi, height) with
+// phi = 2 pi u2 in the same frame, so only radius and height depend on the
+// branch: written this way a wave whose lanes differ in material runs the
+// angle and the frame once.
+ATX_HD f3 sample_next(f3 N, bool ggx, float roughness, uint32_t& seed)
 {
     const float u1 = pcg_float(seed);
     const float u2 = pcg_float(seed);
-    const float r = sqrt_cr(u1);
-    const float theta = 2.0f * kPi * u2;
-    float s, c;
-    sincos_(theta, s, c);
-    return to_world(N, r * c, r * s, sqrt_cr(1.0f - u1));
-}
-
-ATX_HD f3 sample_ggx(f3 N, float roughness, uint32_t& seed)
-{
-    const float u1 = pcg_float(seed);
-    const float u2 = pcg_float(seed);
-    const float a = roughness * roughness;
-    const float cos_theta = sqrt_cr((1.0f - u1) / (1.0f + (a * a - 1.0f) * u1));
-    const float sin_theta = sqrt_cr(1.0f - cos_theta * cos_theta);
+    float radius, height;
+    if (ggx) {
+        const float a = roughness * roughness;
+        const float cos_theta = sqrt_cr((1.0f - u1) / (1.0f + (a * a - 1.0f) * u1));
+        radius = sqrt_cr(1.0f - cos_theta * cos_theta);
+        height = cos_theta;
+    } else {
+        radius = sqrt_cr(u1);
+        height = sqrt_cr(1.0f - u1);
+    }
     const float phi = 2.0f * kPi * u2;
     float s, c;
     sincos_(phi, s, c);
-    return to_world(N, sin_theta * c, sin_theta * s, cos_theta);
+    return to_world(N, radius * c, radius * s, height);
 }
 
 // ---- display (Engine/include/Renderer.h:70-78) -------------------------------

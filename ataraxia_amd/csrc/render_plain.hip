@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_render_plain(PlainArgs a)
             if (pcg_float(seed) > p)
                 break;
             tp = divs(tp, p);
-            rd = metallic > 0.0f ? sample_ggx(nrm, roughness, seed) : sample_cosine(nrm, seed);
+            rd = sample_next(nrm, metallic > 0.0f, roughness, seed);
         }
 
         // kernelRender accumulation and display (Renderer.cu:164-168).  Frame 1

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 end_path = true;
             } else {
                 tp = divs(tp, p);
-                rd = m1.w > 0.0f ? sample_ggx(nrm, m0.w, seed) : sample_cosine(nrm, seed);
+                rd = sample_next(nrm, m1.w > 0.0f, m0.w, seed);
                 ++bounce;
                 if (bounce >= S->max_bounces) {
                     end_path = true;
