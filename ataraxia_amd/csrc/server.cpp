@@ -80,6 +80,11 @@ struct Server {
     int device = 0;
 };
 
+static double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 static double now_s(const Server* s)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->epoch).count();
@@ -166,128 +171,197 @@ static void finish(Server* s, Request& r, bool ok)
     s->cv_done.notify_all();
 }
 
+// Picks queued requests (highest response ratio first) while the resident
+// ones fill less than a launch and the budget allows, and makes them
+// resident; they join the next launch.
+static void admit_queued(Server* s)
+{
+    std::vector<std::shared_ptr<Request>> newly;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const uint32_t F = s->cfg.frames_per_tick;
+        const uint64_t target =
+            s->cfg.tick_mpaths == UINT32_MAX ? UINT64_MAX : (uint64_t)s->cfg.tick_mpaths * 1000000ull;
+        uint64_t tick_paths = 0;
+        for (const auto& r : s->active) tick_paths += tick_paths_of(*r, F);
+        // samples per second: measured, or a nominal rate before the first tick
+        const double rate = s->busy_s > 0.0 ? s->rendered / s->busy_s : 4e9;
+        const double now = now_s(s);
+        while (!s->queue.empty()) {
+            const bool idle = s->active.empty() && newly.empty();
+            if (!idle && tick_paths >= target) break;
+            if (s->cfg.max_active && s->active.size() + newly.size() >= s->cfg.max_active) break;
+            size_t pick = 0;
+            double best = -1.0;
+            for (size_t k = 0; k < s->queue.size(); ++k) {
+                const Request& q = *s->queue[k];
+                const double est = std::max(1e-6, (double)q.cam.width * q.cam.height * q.spp / rate);
+                const double ratio = (now - q.t_submit + est) / est;
+                if (ratio > best) {
+                    best = ratio;
+                    pick = k;
+                }
+            }
+            auto r = s->queue[pick];
+            if (s->stats.hbm_in_use + r->device_bytes > s->cfg.hbm_budget_bytes) break;
+            if (tick_paths + tick_paths_of(*r, F) >= kMaxTickPaths) break;   // 32-bit job indices
+            s->stats.hbm_in_use += r->device_bytes;
+            s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
+            tick_paths += tick_paths_of(*r, F);
+            newly.push_back(r);
+            s->queue.erase(s->queue.begin() + (ptrdiff_t)pick);
+        }
+    }
+    if (newly.empty()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto& r : newly) {
+        if (admit(s, *r)) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            r->state = ATX_REQ_RUNNING;
+            r->t_admit = now_s(s);
+            s->stats.admitted++;
+            s->active.push_back(r);
+        } else {
+            finish(s, *r, false);
+        }
+    }
+    const double ms = ms_since(t0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.admit_ms += ms;
+}
+
+// Enqueues one launch over the resident requests (listed in `in_tick`) on the
+// worker's stream; does not wait for it.
+static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std::shared_ptr<Request>>& in_tick)
+{
+    const uint32_t F = s->cfg.frames_per_tick;
+    const uint32_t P = s->cfg.chunk_px;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        in_tick = s->active;
+    }
+    table.clear();
+    for (auto& r : in_tick) {
+        WfScene w;
+        fill_wf_scene(r->geo, r->settings, r->cam, r->accum.as<float4>(),
+                      s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(), s->cfg.kahan != 0,
+                      r->frames_done + 1u, std::min(F, r->spp - r->frames_done), (int)s->cfg.accel, &w);
+        w.dirs = r->dirs.as<float4>();
+        table.push_back(w);
+    }
+    WfPass wp{};
+    bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
+                                 sizeof(float4)) == hipSuccess;
+    if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
+    ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
+         hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice,
+                        s->stream) == hipSuccess &&
+         launch_zero_u32(s->queue_ctr.as<unsigned int>(), 1, s->stream) == hipSuccess &&
+         hipEventRecord(s->ev0, s->stream) == hipSuccess;
+    if (ok) {
+        wp.scenes = s->table.as<WfScene>();
+        wp.n_scenes = (uint32_t)table.size();
+        wp.queue = s->queue_ctr.as<unsigned int>();
+        wp.counters = s->counters.as<unsigned long long>();
+        const bool flat = wf_flat_only(table.data(), table.size());
+        ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
+             hipEventRecord(s->ev1, s->stream) == hipSuccess;
+    }
+    return ok;
+}
+
+// Waits for the launch, advances the frames of the requests that took part
+// and moves the finished ones to `done` (their results are final).
+static void end_tick(Server* s, bool ok, const std::vector<WfScene>& table,
+                     const std::vector<std::shared_ptr<Request>>& in_tick, std::vector<std::shared_ptr<Request>>& done)
+{
+    const uint32_t F = s->cfg.frames_per_tick;
+    const auto t_wait0 = std::chrono::steady_clock::now();
+    ok = hipStreamSynchronize(s->stream) == hipSuccess && ok;
+    const double wms = ms_since(t_wait0);
+    float ms = 0.0f;
+    if (ok) (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    if (ok) {
+        s->busy_s += ms * 1e-3;
+        for (const WfScene& w : table) s->rendered += (double)w.width * w.height * w.frames;
+    }
+    size_t free_b = 0, total_b = 0;
+    if ((s->stats.ticks & 63u) == 0u) (void)hipMemGetInfo(&free_b, &total_b);
+    std::vector<std::shared_ptr<Request>> failed;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.ticks++;
+        s->stats.gpu_busy_ms += ms;
+        s->stats.wait_ms += wms;
+        if (total_b) s->stats.device_used_high_water = std::max<uint64_t>(s->stats.device_used_high_water, total_b - free_b);
+        for (auto& r : in_tick) {
+            if (!ok) {
+                failed.push_back(r);
+                continue;
+            }
+            r->frames_done += std::min(F, r->spp - r->frames_done);
+            if (r->frames_done >= r->spp) done.push_back(r);
+        }
+        auto gone = [&](const std::shared_ptr<Request>& r) {
+            return std::find(done.begin(), done.end(), r) != done.end() ||
+                   std::find(failed.begin(), failed.end(), r) != failed.end();
+        };
+        s->active.erase(std::remove_if(s->active.begin(), s->active.end(), gone), s->active.end());
+    }
+    for (auto& r : failed) {
+        r->error = "launch failed";
+        finish(s, *r, false);
+    }
+}
+
+// Copies finished requests' results to host memory and releases their
+// device buffers.
+static void retire(Server* s, std::vector<std::shared_ptr<Request>>& done)
+{
+    if (done.empty()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto& r : done) finish(s, *r, true);
+    done.clear();
+    const double ms = ms_since(t0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.retire_ms += ms;
+}
+
+// The worker: each pass enqueues a launch over the resident requests and,
+// while it runs, admits queued requests for the next launch and retires the
+// requests the previous launch finished (neither group takes part in the
+// running launch), then waits for the launch.  Admission, result copies and
+// releases therefore overlap rendering instead of idling the device.
 static void worker_main(Server* s)
 {
     (void)hipSetDevice(s->device);
     std::vector<WfScene> table;
+    std::vector<std::shared_ptr<Request>> in_tick, done;
     for (;;) {
-        // ---- admission (FIFO, HBM budget) ----
-        std::vector<std::shared_ptr<Request>> newly;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty() || !s->active.empty(); });
+            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->active.empty() || !done.empty(); });
             if (s->stop) break;
-            const uint32_t F = s->cfg.frames_per_tick;
-            const uint64_t target =
-                s->cfg.tick_mpaths == UINT32_MAX ? UINT64_MAX : (uint64_t)s->cfg.tick_mpaths * 1000000ull;
-            uint64_t tick_paths = 0;
-            for (const auto& r : s->active) tick_paths += tick_paths_of(*r, F);
-            // samples per second: measured, or a nominal rate before the first tick
-            const double rate = s->busy_s > 0.0 ? s->rendered / s->busy_s : 4e9;
-            const double now = now_s(s);
-            while (!s->queue.empty()) {
-                const bool idle = s->active.empty() && newly.empty();
-                if (!idle && tick_paths >= target) break;
-                if (s->cfg.max_active && s->active.size() + newly.size() >= s->cfg.max_active) break;
-                size_t pick = 0;
-                double best = -1.0;
-                for (size_t k = 0; k < s->queue.size(); ++k) {
-                    const Request& q = *s->queue[k];
-                    const double est = std::max(1e-6, (double)q.cam.width * q.cam.height * q.spp / rate);
-                    const double ratio = (now - q.t_submit + est) / est;
-                    if (ratio > best) {
-                        best = ratio;
-                        pick = k;
-                    }
-                }
-                auto r = s->queue[pick];
-                if (s->stats.hbm_in_use + r->device_bytes > s->cfg.hbm_budget_bytes) break;
-                if (tick_paths + tick_paths_of(*r, F) >= kMaxTickPaths) break;   // 32-bit job indices
-                s->stats.hbm_in_use += r->device_bytes;
-                s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
-                tick_paths += tick_paths_of(*r, F);
-                newly.push_back(r);
-                s->queue.erase(s->queue.begin() + (ptrdiff_t)pick);
-            }
         }
-        for (auto& r : newly) {
-            if (admit(s, *r)) {
-                std::lock_guard<std::mutex> lk(s->mu);
-                r->state = ATX_REQ_RUNNING;
-                r->t_admit = now_s(s);
-                s->stats.admitted++;
-                s->active.push_back(r);
-            } else {
-                finish(s, *r, false);
-            }
-        }
-        if (s->active.empty()) continue;
-
-        // ---- one launch over every admitted request ----
-        const uint32_t F = s->cfg.frames_per_tick;
-        const uint32_t P = s->cfg.chunk_px;
-        table.clear();
-        for (auto& r : s->active) {
-            WfScene w;
-            fill_wf_scene(r->geo, r->settings, r->cam, r->accum.as<float4>(),
-                          s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(),
-                          s->cfg.kahan != 0, r->frames_done + 1u, std::min(F, r->spp - r->frames_done),
-                          (int)s->cfg.accel, &w);
-            w.dirs = r->dirs.as<float4>();
-            table.push_back(w);
-        }
-        WfPass wp{};
-        bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
-                                     sizeof(float4)) == hipSuccess;
-        if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
-        ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
-                  hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene),
-                                 hipMemcpyHostToDevice, s->stream) == hipSuccess &&
-                  launch_zero_u32(s->queue_ctr.as<unsigned int>(), 1, s->stream) == hipSuccess &&
-                  hipEventRecord(s->ev0, s->stream) == hipSuccess;
-        if (ok) {
-            wp.scenes = s->table.as<WfScene>();
-            wp.n_scenes = (uint32_t)table.size();
-            wp.queue = s->queue_ctr.as<unsigned int>();
-            wp.counters = s->counters.as<unsigned long long>();
-            const bool flat = wf_flat_only(table.data(), table.size());
-            ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
-                 hipEventRecord(s->ev1, s->stream) == hipSuccess && hipStreamSynchronize(s->stream) == hipSuccess;
-        }
-        float ms = 0.0f;
-        if (ok) (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
-        if (ok) {
-            s->busy_s += ms * 1e-3;
-            for (const WfScene& w : table) s->rendered += (double)w.width * w.height * w.frames;
-        }
-        size_t free_b = 0, total_b = 0;
-        (void)hipMemGetInfo(&free_b, &total_b);
+        bool idle;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->stats.ticks++;
-            s->stats.gpu_busy_ms += ms;
-            s->stats.device_used_high_water = std::max<uint64_t>(s->stats.device_used_high_water, total_b - free_b);
+            idle = s->active.empty();
         }
-        // ---- advance, retire finished requests ----
-        std::vector<std::shared_ptr<Request>> still;
-        for (auto& r : s->active) {
-            if (!ok) {
-                r->error = "launch failed";
-                finish(s, *r, false);
-                continue;
-            }
-            bool complete;
-            {
-                std::lock_guard<std::mutex> lk(s->mu);
-                r->frames_done += std::min(F, r->spp - r->frames_done);
-                complete = r->frames_done >= r->spp;
-            }
-            if (complete) finish(s, *r, true);
-            else still.push_back(r);
+        if (idle) admit_queued(s);   // nothing resident: admit before launching
+        bool launched = false, ok = false;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            launched = !s->active.empty();
         }
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->active.swap(still);
+        if (launched) {
+            ok = launch_tick(s, table, in_tick);
+            if (ok) admit_queued(s);   // joins the next launch
+        }
+        retire(s, done);
+        if (launched) end_tick(s, ok, table, in_tick, done);
     }
+    retire(s, done);
 }
 
 static Server* cast(atx_server* h) { return reinterpret_cast<Server*>(h); }
