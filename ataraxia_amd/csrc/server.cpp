@@ -69,6 +69,12 @@ struct Server {
     uint64_t next_id = 1;
     bool stop = false;
     std::thread worker;
+    // finished requests whose results the retire thread copies out
+    std::thread retirer;
+    std::deque<std::shared_ptr<Request>> retire_q;
+    std::vector<std::shared_ptr<Request>> release_q;   // retired, device buffers still held
+    std::condition_variable cv_retire;
+    bool retire_stop = false;
     atx_server_stats stats{};
     // worker-only device resources
     hipStream_t stream = nullptr;
@@ -141,8 +147,37 @@ static bool admit(Server* s, Request& r)
     return true;
 }
 
-static void finish(Server* s, Request& r, bool ok)
+// Releases a request's device buffers.  A release waits for the device to
+// be idle, so only the worker calls it, between launches.
+static void release_device(Server* s, Request& r)
 {
+    r.geo = DevScene();
+    r.accum.release();
+    r.comp.release();
+    r.image.release();
+    r.dirs.release();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.hbm_in_use -= r.device_bytes;
+}
+
+// Releases the device buffers of requests the retire thread has finished
+// (worker, with the device idle).
+static void release_returned(Server* s)
+{
+    std::vector<std::shared_ptr<Request>> rel;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        rel.swap(s->release_q);
+    }
+    for (auto& r : rel) release_device(s, *r);
+}
+
+// Copies a finished request's results to host memory (ok) and marks it done
+// or failed.  The worker releases the device buffers at once (release_now);
+// the retire thread hands the request back to the worker for that.
+static void finish(Server* s, const std::shared_ptr<Request>& rp, bool ok, bool release_now)
+{
+    Request& r = *rp;
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     if (ok) {
         r.rgba.resize(npx);
@@ -153,15 +188,11 @@ static void finish(Server* s, Request& r, bool ok)
             r.error = "result copy failed";
         }
     }
-    r.geo = DevScene();
-    r.accum.release();
-    r.comp.release();
-    r.image.release();
-    r.dirs.release();
+    if (release_now) release_device(s, r);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (!release_now) s->release_q.push_back(rp);
     r.state = ok ? ATX_REQ_DONE : ATX_REQ_FAILED;
     r.t_done = now_s(s);
-    s->stats.hbm_in_use -= r.device_bytes;
     if (ok) {
         s->stats.completed++;
         s->stats.samples += (uint64_t)npx * r.spp;
@@ -222,7 +253,7 @@ static void admit_queued(Server* s)
             s->stats.admitted++;
             s->active.push_back(r);
         } else {
-            finish(s, *r, false);
+            finish(s, r, false, true);
         }
     }
     const double ms = ms_since(t0);
@@ -310,28 +341,53 @@ static void end_tick(Server* s, bool ok, const std::vector<WfScene>& table,
     }
     for (auto& r : failed) {
         r->error = "launch failed";
-        finish(s, *r, false);
+        finish(s, r, false, true);
     }
+    release_returned(s);   // the device is idle
 }
 
-// Copies finished requests' results to host memory and releases their
-// device buffers.
+// Hands finished requests to the retire thread, which copies their results
+// to host memory and releases their device buffers.
 static void retire(Server* s, std::vector<std::shared_ptr<Request>>& done)
 {
     if (done.empty()) return;
-    const auto t0 = std::chrono::steady_clock::now();
-    for (auto& r : done) finish(s, *r, true);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto& r : done) s->retire_q.push_back(r);
+    }
     done.clear();
-    const double ms = ms_since(t0);
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->stats.retire_ms += ms;
+    s->cv_retire.notify_one();
+}
+
+static void retirer_main(Server* s)
+{
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_retire.wait(lk, [s] { return s->retire_stop || !s->retire_q.empty(); });
+            if (s->retire_q.empty()) break;   // stopping, nothing left
+            r = s->retire_q.front();
+            s->retire_q.pop_front();
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        finish(s, r, true, false);
+        const double ms = ms_since(t0);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stats.retire_ms += ms;
+        }
+        s->cv_work.notify_one();   // its memory can be released and reused
+    }
 }
 
 // The worker: each pass enqueues a launch over the resident requests and,
-// while it runs, admits queued requests for the next launch and retires the
-// requests the previous launch finished (neither group takes part in the
-// running launch), then waits for the launch.  Admission, result copies and
-// releases therefore overlap rendering instead of idling the device.
+// while it runs, admits queued requests for the next launch and hands the
+// requests the previous launch finished to the retire thread (neither group
+// takes part in the running launch), then waits for the launch.  Uploads,
+// result copies and releases therefore overlap rendering instead of idling
+// the device.
 static void worker_main(Server* s)
 {
     (void)hipSetDevice(s->device);
@@ -348,7 +404,10 @@ static void worker_main(Server* s)
             std::lock_guard<std::mutex> lk(s->mu);
             idle = s->active.empty();
         }
-        if (idle) admit_queued(s);   // nothing resident: admit before launching
+        if (idle) {
+            release_returned(s);   // nothing runs: free retired requests' memory, then admit
+            admit_queued(s);
+        }
         bool launched = false, ok = false;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -359,7 +418,15 @@ static void worker_main(Server* s)
             if (ok) admit_queued(s);   // joins the next launch
         }
         retire(s, done);
-        if (launched) end_tick(s, ok, table, in_tick, done);
+        if (launched) {
+            end_tick(s, ok, table, in_tick, done);
+        } else {
+            // nothing could be admitted: wait until the retire thread hands
+            // memory back (or something else changes) instead of spinning
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait_for(lk, std::chrono::milliseconds(5),
+                                [s] { return s->stop || !s->release_q.empty() || s->queue.empty(); });
+        }
     }
     retire(s, done);
 }
@@ -409,9 +476,22 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     s->epoch = std::chrono::steady_clock::now();
     Server* raw = s.release();
     try {
+        raw->retirer = std::thread(retirer_main, raw);
+    } catch (...) {
+        set_error("cannot start the server threads");
+        delete raw;
+        return ATX_ERR_STATE;
+    }
+    try {
         raw->worker = std::thread(worker_main, raw);
     } catch (...) {
-        set_error("cannot start the server thread");
+        {
+            std::lock_guard<std::mutex> lk(raw->mu);
+            raw->retire_stop = true;
+        }
+        raw->cv_retire.notify_all();
+        raw->retirer.join();
+        set_error("cannot start the server threads");
         delete raw;
         return ATX_ERR_STATE;
     }
@@ -429,6 +509,12 @@ int atx_server_destroy(atx_server* h)
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->retire_stop = true;   // the retire thread drains its queue first
+    }
+    s->cv_retire.notify_all();
+    if (s->retirer.joinable()) s->retirer.join();
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->active.clear();
     s->queue.clear();

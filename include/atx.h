@@ -239,7 +239,8 @@ typedef struct atx_server_stats {
     uint64_t device_used_high_water; /* max device memory in use (hipMemGetInfo) */
     double gpu_busy_ms;            /* HIP-event time of all launches            */
     double admit_ms;               /* worker time admitting requests (uploads)  */
-    double retire_ms;              /* worker time retiring them (result copies) */
+    double retire_ms;              /* time retiring them (result copies), on a
+                                      thread of its own                         */
     double wait_ms;                /* worker time waiting for launches to end   */
 } atx_server_stats;
 
