@@ -1,7 +1,7 @@
 // server.cpp -- continuous-batching render service on one GPU.
 //
 // Submitters copy their scene into a request and return at once.  One worker
-// thread owns the device: between launches it admits queued requests while
+// thread drives the device: it admits queued requests while
 // the admitted ones fill less than tick_mpaths million paths per launch (so
 // the device stays full without every queued request sharing it) and their
 // device buffers fit the HBM budget (geometry, accumulation, optional Kahan
@@ -12,11 +12,14 @@
 // a long one's priority grows while it waits.  Every admitted request is
 // rendered together with the others by one launch of the persistent
 // wavefront kernel per tick (frames_per_tick frames per request, each request
-// at its own frame index).  Requests that arrive while a launch runs
-// join at the next launch.  A request that reaches its spp is copied to host
-// memory and its device buffers are released immediately, which makes room
-// for the next admissions.  Per-request timestamps (submit, admit, done) give
-// the latency distribution.
+// at its own frame index).  Requests that arrive while a launch runs join at
+// the next launch: the worker enqueues a tick and admits the next tick's
+// requests while it runs.  Requests that reach their spp go to a retire
+// thread, which copies their results to host memory and marks them done; the
+// worker releases their device buffers once the running tick has ended (a
+// release waits for the device), which makes room for the next admissions.
+// Per-request timestamps (submit, admit, done) give the latency
+// distribution.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
