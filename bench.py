@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="batch_mix: skip the plain-kernel comparison")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="batch_mix: scale every scene's spp")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: Poisson arrival rate (requests/s)")
-    ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--frames-per-tick", type=int, default=0,
                     help="serve: frames per request per launch (0: the server's default)")
     ap.add_argument("--tick-mpaths", type=int, default=0,
