@@ -101,9 +101,14 @@ def cpu_baseline(sc, target_s):
         if dt >= target_s:
             break
     samples = n * frames
+    # one more frame on a single thread: the per-core rate
+    t1 = time.perf_counter()
+    O.render_frames_mt(cam, osc, st, frames + 1, 1, 1, accum=acc)
+    dt1 = time.perf_counter() - t1
     return {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full frame(s) of the same 1080p workload ({samples} samples, {rays} rays) "
-                      f"in {dt:.2f} s on {threads} threads"}
+                      f"in {dt:.2f} s on {threads} threads; one more frame on 1 thread in {dt1:.2f} s",
+            "single_thread_value": n / dt1}
 
 
 def time_single_scene(sc, kernel, accel, steps, warmup):
