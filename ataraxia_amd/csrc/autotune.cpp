@@ -95,7 +95,7 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
     }
     batch_set_subset(b, sample);
     double best_rate = -1.0;
-    atx_batch_tuning best{kDefaultChunkPx, 0, 1, kDefaultFramesPerLaunch};
+    atx_batch_tuning best{0, 0, 1, kDefaultFramesPerLaunch};
     auto try_one = [&](const atx_batch_tuning& t) -> int {
         atx_batch_tuning tt = t;
         int e = atx_batch_set_tuning(h, &tt);

@@ -48,6 +48,7 @@ struct Batch {
     std::vector<uint32_t> launch_off;     // first table entry of launch l
     std::vector<uint32_t> launch_scenes;  // scenes taking part in launch l
     std::vector<uint32_t> launch_chunks;  // chunk ids of launch l
+    std::vector<uint32_t> launch_chunk_px;  // jobs per chunk of launch l
     uint32_t launches = 0, chunk_px = 0, frames_per_launch = 0, grid = 0, max_spp = 0;
     bool flat_only = false;   // no scene has a tree: the flat-loop kernel instance
     bool kahan = false;
@@ -77,7 +78,7 @@ static int enqueue_launches(Batch* b)
         wp.scenes = b->d_table.as<WfScene>() + b->launch_off[l];
         wp.n_scenes = b->launch_scenes[l];
         wp.total_chunks = b->launch_chunks[l];
-        wp.chunk_px = b->chunk_px;
+        wp.chunk_px = b->launch_chunk_px[l];
         wp.total_px = b->launch_px[l];
         wp.queue = b->d_queue.as<unsigned int>() + l;
         wp.counters = b->counters.as<unsigned long long>();
@@ -100,7 +101,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     const size_t n = b->order.size();
     std::stable_sort(b->order.begin(), b->order.end(),
                      [b](uint32_t x, uint32_t y) { return b->scenes[x]->spp > b->scenes[y]->spp; });
-    b->chunk_px = b->tuning.chunk_px ? b->tuning.chunk_px : kDefaultChunkPx;
+    b->chunk_px = b->tuning.chunk_px;   // 0: auto_chunk_px per launch
     b->frames_per_launch = b->tuning.frames_per_launch ? b->tuning.frames_per_launch : kDefaultFramesPerLaunch;
     b->kahan = b->accumulator == ATX_ACCUM_KAHAN;
     b->flat_only = batch_flat_only(b);
@@ -112,6 +113,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     b->launch_off.clear();
     b->launch_scenes.clear();
     b->launch_chunks.clear();
+    b->launch_chunk_px.clear();
     b->launch_px.clear();
     // Launch (f0, group) renders frames [f0 + 1, f0 + F] of one group of the
     // scenes whose spp exceeds f0: consecutive scenes in spp order whose
@@ -157,13 +159,17 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
         most = std::max(most, launch_samples(table.data() + b->launch_off[l], b->launch_scenes[l]));
     ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
+        const WfScene* lt = table.data() + b->launch_off[l];
+        const uint32_t P = b->chunk_px ? b->chunk_px
+                                       : auto_chunk_px(launch_samples(lt, b->launch_scenes[l]), b->grid, b->flat_only);
         WfPass lp{};
-        if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], b->chunk_px,
+        if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], P,
                            b->d_samples.as<float4>(), &lp)) {
             set_error("batch: a launch of %u frames holds 2^31 or more paths; lower frames_per_launch", F);
             return ATX_ERR_INVALID;
         }
         b->launch_chunks.push_back(lp.total_chunks);
+        b->launch_chunk_px.push_back(P);
         b->launch_px.push_back(lp.total_px);
     }
     b->launches = (uint32_t)b->launch_off.size();
@@ -388,7 +394,6 @@ int atx_batch_get_tuning(const atx_batch* h, atx_batch_tuning* t)
     const Batch* b = reinterpret_cast<const Batch*>(h);
     ATX_CHECK_ARG(b && t, "NULL argument");
     *t = b->tuning;
-    if (!t->chunk_px) t->chunk_px = b->chunk_px ? b->chunk_px : kDefaultChunkPx;
     if (!t->frames_per_launch) t->frames_per_launch = kDefaultFramesPerLaunch;
     if (!t->workgroups)
         t->workgroups = b->grid ? b->grid : wavefront_full_grid(batch_flat_only(b));

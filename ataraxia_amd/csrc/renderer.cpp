@@ -29,11 +29,11 @@ struct Renderer {
     std::vector<hipEvent_t> wf_ev;   // per wavefront launch: path kernel start, end
     DevScene geo;
     bool has_scene = false;
-    int kernel = ATX_KERNEL_PLAIN;
+    int kernel = ATX_KERNEL_WAVEFRONT;
     int accel = ATX_ACCEL_BVH;
     DevBuf wf_table, wf_queue, wf_samples, wf_dirs;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
-    uint32_t wf_chunk = kDefaultChunkPx;
+    uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kDefaultFramesPerLaunch;
     DevBuf accum, comp, image, counters;
     uint32_t width = 0, height = 0;
@@ -104,9 +104,9 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     // wavefront mode: launches of up to wf_frames frames each (one frame per
     // launch when accumulation is off: the frame index then stays 1)
     std::vector<WfScene> table;
-    std::vector<uint32_t> launch_frames;
-    const uint32_t chunk = r->wf_chunk;
+    std::vector<uint32_t> launch_frames, launch_chunk;
     const bool wf = r->kernel == ATX_KERNEL_WAVEFRONT && frames > 0;
+    WfLaunch shape{0, false};
     if (wf) {
         uint32_t fi = r->frame_index, left = frames;
         const uint32_t fit = (uint32_t)std::max<size_t>(1, kSampleBufferBytes / (npx * sizeof(float4)));
@@ -129,9 +129,14 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         size_t most = 0;
         for (const WfScene& e : table) most = std::max(most, launch_samples(&e, 1));
         ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
+        shape.flat_only = wf_flat_only(table.data(), table.size());
+        shape.workgroups = wavefront_full_grid(shape.flat_only);
         for (WfScene& e : table) {
+            const uint32_t P = r->wf_chunk ? r->wf_chunk
+                                           : auto_chunk_px(launch_samples(&e, 1), shape.workgroups, shape.flat_only);
+            launch_chunk.push_back(P);
             WfPass lp{};
-            if (!layout_launch(&e, 1, chunk, r->wf_samples.as<float4>(), &lp)) {
+            if (!layout_launch(&e, 1, P, r->wf_samples.as<float4>(), &lp)) {
                 set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
                 return ATX_ERR_INVALID;
             }
@@ -155,14 +160,12 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) {
         // the camera is the same for every launch of this call
         ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
-        const bool flat = wf_flat_only(table.data(), table.size());
-        const WfLaunch shape{wavefront_full_grid(flat), flat};
         for (size_t l = 0; l < table.size(); ++l) {
             WfPass wp{};
             wp.scenes = r->wf_table.as<WfScene>() + l;
             wp.n_scenes = 1;
             wp.total_chunks = table[l].n_chunks;
-            wp.chunk_px = chunk;
+            wp.chunk_px = launch_chunk[l];
             wp.total_px = table[l].width * table[l].height;
             wp.queue = r->wf_queue.as<unsigned int>() + l;
             wp.counters = a.counters;

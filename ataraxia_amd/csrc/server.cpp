@@ -269,7 +269,6 @@ static void admit_queued(Server* s)
 static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std::shared_ptr<Request>>& in_tick)
 {
     const uint32_t F = s->cfg.frames_per_tick;
-    const uint32_t P = s->cfg.chunk_px;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         in_tick = s->active;
@@ -286,6 +285,10 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
     WfPass wp{};
     bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
                                  sizeof(float4)) == hipSuccess;
+    const bool flat = wf_flat_only(table.data(), table.size());
+    const WfLaunch shape{flat ? s->grid_flat : s->grid, flat};
+    const uint32_t P = s->cfg.chunk_px ? s->cfg.chunk_px
+                                       : auto_chunk_px(launch_samples(table.data(), table.size()), shape.workgroups, flat);
     if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
     ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
          hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice,
@@ -297,8 +300,7 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
         wp.n_scenes = (uint32_t)table.size();
         wp.queue = s->queue_ctr.as<unsigned int>();
         wp.counters = s->counters.as<unsigned long long>();
-        const bool flat = wf_flat_only(table.data(), table.size());
-        ok = launch_wavefront(wp, WfLaunch{flat ? s->grid_flat : s->grid, flat}, s->stream) == hipSuccess &&
+        ok = launch_wavefront(wp, shape, s->stream) == hipSuccess &&
              hipEventRecord(s->ev1, s->stream) == hipSuccess;
     }
     return ok;
@@ -456,7 +458,6 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = kDefaultFramesPerLaunch;
     if (!s->cfg.tick_mpaths) s->cfg.tick_mpaths = 32;
-    if (!s->cfg.chunk_px) s->cfg.chunk_px = kDefaultChunkPx;
     ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");
     s->cfg.kahan = s->cfg.kahan ? 1u : 0u;

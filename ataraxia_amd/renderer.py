@@ -83,7 +83,7 @@ class Renderer:
         check(lib.atx_renderer_set_accumulator(self._h, kind))
 
     def set_kernel(self, kind: int) -> None:
-        """ATX_KERNEL_PLAIN (one lane per pixel) or ATX_KERNEL_WAVEFRONT."""
+        """ATX_KERNEL_WAVEFRONT (the default, persistent) or ATX_KERNEL_PLAIN (one lane per pixel)."""
         check(lib.atx_renderer_set_kernel(self._h, kind))
 
     def set_accel(self, accel: int) -> None:

@@ -144,8 +144,9 @@ int atx_renderer_render_debug(atx_renderer* r, const atx_camera* cam, float* rad
                               int32_t* hit_ids, int32_t* shadow_ids, uint32_t max_bounces);
 
 /* Kernel used by atx_renderer_render*: PLAIN = one lane per pixel, the
- * reference's per-pixel loop; WAVEFRONT = the persistent path-regenerating
- * kernel (results are identical, only the scheduling differs).             */
+ * reference's per-pixel loop; WAVEFRONT (default) = the persistent
+ * path-regenerating kernel (results are identical, only the scheduling
+ * differs; it is faster also at one frame per call).                       */
 #define ATX_KERNEL_PLAIN 0
 #define ATX_KERNEL_WAVEFRONT 1
 int atx_renderer_set_kernel(atx_renderer* r, int kind);
@@ -170,7 +171,8 @@ typedef struct atx_scene_desc {
 } atx_scene_desc;
 
 typedef struct atx_batch_tuning {
-    uint32_t chunk_px;          /* paths per work chunk: multiple of 64, <= 1024 (0 = 512)  */
+    uint32_t chunk_px;          /* paths per work chunk: multiple of 64, <= 1024
+                                   (0 = chosen per launch from its paths per wave)         */
     uint32_t workgroups;        /* persistent grid, 256-lane workgroups (0 = fill device) */
     uint32_t use_graph;         /* capture all launches in a HIP graph and replay it       */
     uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 64)           */
@@ -221,7 +223,7 @@ int atx_batch_get_stats(const atx_batch* b, atx_render_stats* stats);
 typedef struct atx_server_config {
     uint64_t hbm_budget_bytes;  /* device bytes for request buffers (0 = half of free HBM) */
     uint32_t frames_per_tick;   /* frames each request advances per launch (0 = 64)         */
-    uint32_t chunk_px;          /* paths per work chunk (0 = 512)                           */
+    uint32_t chunk_px;          /* paths per work chunk (0 = chosen per launch)             */
     uint32_t max_active;        /* cap on resident requests (0 = budget only)               */
     uint32_t kahan;             /* 1 = compensated accumulation                             */
     uint32_t accel;             /* ATX_ACCEL_BVH (0) or ATX_ACCEL_NONE                      */

@@ -162,10 +162,12 @@ def test_large_scene_global_memory_path(kernel):
     run_parity(sc, 2, kernel=kernel)
 
 
-def test_accumulation_off_resets_every_frame():
+@KERNELS
+def test_accumulation_off_resets_every_frame(kernel):
     sc = scene_from_camera_overrides(plain_cpu_scene(), 32, 18)
     sc.settings = Settings(False, False, 5)
     with Renderer() as r:
+        r.set_kernel(kernel)
         r.on_resize(32, 18)
         r.set_settings(sc.settings)
         r.set_scene(sc)
@@ -177,7 +179,8 @@ def test_accumulation_off_resets_every_frame():
     assert np.array_equal(acc, rad)
 
 
-def test_full_1080p_sampled_pixels_and_determinism():
+@KERNELS
+def test_full_1080p_sampled_pixels_and_determinism(kernel):
     """Full-size workload (1920x1080, 257 spheres, 8 lights, 8 bounces): the
     oracle checks 4096 sampled pixels per frame; the whole image must be
     bit-identical across two renderers (determinism)."""
@@ -192,8 +195,10 @@ def test_full_1080p_sampled_pixels_and_determinism():
     rng = np.random.default_rng(5)
     pix = rng.choice(cam.width * cam.height, size=4096, replace=False)
     outs = []
+    print(f"case plain_1080p kernel={kernel}", flush=True)
     for rep in range(2):
         with Renderer() as r:
+            r.set_kernel(kernel)
             r.on_resize(cam.width, cam.height)
             r.set_settings(st)
             r.set_scene(sc)

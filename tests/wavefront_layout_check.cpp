@@ -1,6 +1,7 @@
 // Host-side checks of the persistent kernel's launch layout
 // (ataraxia_amd/csrc/wavefront_host.h): chunk ids, pixel and sample offsets,
-// launch totals, the 32-bit guard, and the flat-loop test of a table.
+// launch totals, the 32-bit guard, the flat-loop test of a table, and the
+// chunk size chosen per launch (auto_chunk_px).
 // Built host-only (hipcc --offload-host-only) by tests/test_wavefront_host.py.
 #include <cstdio>
 #include <vector>
@@ -64,6 +65,24 @@ int main()
     std::vector<WfScene> one = {scene(1, 1, 1, 0)};
     CHECK(layout_launch(one.data(), one.size(), 64, base, &p));
     CHECK(one[0].n_chunks == 1 && p.total_chunks == 1 && p.total_px == 1);
+
+    // chunk size chosen per launch: 1080p frames on a 256-CU grid
+    // (tree launches 6 workgroups per CU, flat-loop launches 7)
+    const uint64_t f1080 = 1920ull * 1080;
+    CHECK(auto_chunk_px(f1080, 1536, false) == 128);        // 337 jobs per wave
+    CHECK(auto_chunk_px(2 * f1080, 1536, false) == 128);
+    CHECK(auto_chunk_px(8 * f1080, 1536, false) == 256);    // 2700 per wave
+    CHECK(auto_chunk_px(32 * f1080, 1536, false) == 512);
+    CHECK(auto_chunk_px(64 * f1080, 1536, false) == 512);
+    CHECK(auto_chunk_px(f1080, 1792, true) == 64);          // 289 per wave
+    CHECK(auto_chunk_px(2 * f1080, 1792, true) == 128);
+    CHECK(auto_chunk_px(64 * f1080, 1792, true) == 128);
+    CHECK(auto_chunk_px(1, 0, false) == 128 && auto_chunk_px(0, 1, true) == 64);
+    for (uint64_t jobs : {uint64_t(1), uint64_t(1000), f1080, 64 * f1080, uint64_t(1) << 31})
+        for (bool flat : {false, true}) {
+            const uint32_t c = auto_chunk_px(jobs, 1536, flat);
+            CHECK(c % 64 == 0 && c >= 64 && c <= kMaxChunkPx);
+        }
 
     if (failures == 0) std::printf("wavefront layout: all checks passed\n");
     return failures == 0 ? 0 : 1;
