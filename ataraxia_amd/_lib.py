@@ -99,7 +99,7 @@ class RequestInfo(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays", C.c_uint64), ("sphere_tests", C.c_uint64),
                 ("exact_retraces", C.c_uint64), ("box_tests", C.c_uint64), ("gpu_ms", C.c_double),
-                ("launches", C.c_uint64), ("path_ms", C.c_double)]
+                ("launches", C.c_uint64), ("path_ms", C.c_double), ("reused_rays", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -27,6 +27,7 @@ struct BatchEntry {
     DevScene geo;
     DevBuf accum, comp, image;
     DevBuf dirs;                  // primary ray direction per pixel (written once)
+    DevBuf first;                 // first-bounce cache per pixel (written once)
     bool dirs_ready = false;
     atx_settings settings{};
     atx_camera cam{};
@@ -137,12 +138,18 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
                           b->kahan ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan,
                           f0 + 1u, std::min(F, spp - f0), b->accel, &w);
+            w.dirs = e.dirs.as<float4>();
+            w.first = e.first.as<uint4>();
             if (!e.dirs_ready) {
-                ATX_HIP(e.dirs.reserve((size_t)e.cam.width * e.cam.height * sizeof(float4)));
+                const size_t npx = (size_t)e.cam.width * e.cam.height;
+                ATX_HIP(e.dirs.reserve(npx * sizeof(float4)));
+                ATX_HIP(e.first.reserve(npx * sizeof(uint4)));
+                w.dirs = e.dirs.as<float4>();
+                w.first = e.first.as<uint4>();
                 ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
+                ATX_HIP(launch_first_hits(w, e.first.as<uint4>(), b->stream));
                 e.dirs_ready = true;
             }
-            w.dirs = e.dirs.as<float4>();
             if (e.capture) {
                 w.dbg_radiance = e.dbg_rad.as<float4>();
                 w.dbg_hit = e.dbg_hit.as<int32_t>();
@@ -226,7 +233,8 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
         samples += (uint64_t)e.cam.width * e.cam.height * frames;
     }
     b->stats.samples = samples;
-    b->stats.rays = ctr[kCtrRays];
+    b->stats.rays = ctr[kCtrRays] + ctr[kCtrReused];
+    b->stats.reused_rays = ctr[kCtrReused];
     b->stats.sphere_tests = ctr[kCtrTests];
     b->stats.exact_retraces = ctr[kCtrExact];
     b->stats.box_tests = ctr[kCtrBoxes];

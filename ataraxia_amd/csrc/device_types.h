@@ -45,7 +45,8 @@ enum CounterSlot : uint32_t {
     kCtrExact = 1,         // traces resolved by the exact fallback loop
     kCtrTests = 2,         // ray-sphere tests
     kCtrBoxes = 3,         // ray-box tests of BVH nodes
-    kCtrCount = 4,
+    kCtrReused = 4,        // rays answered from the per-pixel first-bounce cache
+    kCtrCount = 5,
 };
 
 struct PlainArgs {

@@ -81,18 +81,114 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-// Adds the wave's per-lane test counts to the launch's counters and clears
-// them (every lane of the wave calls it).
+// Adds the wave's per-lane counts to the launch's counters and clears them
+// (every lane of the wave calls it).
 __device__ __forceinline__ void flush_counters(unsigned long long* ctr, uint32_t lane, uint32_t& tests,
-                                               uint32_t& boxes)
+                                               uint32_t& boxes, uint32_t& reused)
 {
     const unsigned long long ts = wave_sum(tests);
     const unsigned long long bx = wave_sum(boxes);
+    const unsigned long long ru = wave_sum(reused);
     if (lane == 0) {
         if (ts) atomicAdd(ctr + kCtrTests, ts);
         if (bx) atomicAdd(ctr + kCtrBoxes, bx);
+        if (ru) atomicAdd(ctr + kCtrReused, ru);
     }
-    tests = boxes = 0u;
+    tests = boxes = reused = 0u;
+}
+
+// ---- one bounce of the per-pixel path (Renderer.cu:319-381), in pieces
+// shared by the path kernel and the first-bounce cache, so that both
+// evaluate the same expressions in the same order ----
+
+// Hit point, normal and the origin of the rays that leave the hit
+// (Renderer.cu:325-333).
+__device__ __forceinline__ void hit_frame(float4 sphere, const Hit& h, f3 ro, f3 rd, f3& pos, f3& nrm, f3& so)
+{
+    const f3 center = xyz(sphere);
+    pos = add(sub(ro, center), scale(rd, h.t));
+    nrm = normalize(pos);
+    pos = add(pos, center);
+    so = add(pos, scale(nrm, 0.0001f));
+}
+
+// Direction (normalised) and squared distance from pos to a light
+// (Renderer.cu:343-346).
+__device__ __forceinline__ void light_ray(float4 light_pos, f3 pos, f3& L, float& dist2)
+{
+    L = sub(xyz(light_pos), pos);
+    dist2 = dot(L, L);
+    L = normalize(L);
+}
+
+// The shadow decision on the shadow ray's closest hit (Renderer.cu:348-350):
+// lit unless that hit lies strictly between the surface and the light.
+__device__ __forceinline__ bool lit_by(const Hit& h, float dist2)
+{
+    const float st = h.id < 0 ? -1.0f : h.t;
+    return !(st > 0.0f && st * st < dist2);
+}
+
+// Emission of a continuation hit and the light its shadow ray goes to.
+// Returns true when a shadow ray follows (the scene has lights).
+__device__ __forceinline__ bool shade_hit(CScene* S, const Hit& h, f3 ro, f3 rd, f3 tp, uint32_t seed, f3& col,
+                                          f3& nrm, f3& so, int32_t& mid, uint32_t& lidx, f3& L, float& dist2)
+{
+    f3 pos;
+    hit_frame(as_const(S->sph)[h.id], h, ro, rd, pos, nrm, so);
+    mid = S->sph_mat[h.id];
+    const float4 m2 = S->mat[3 * mid + 2];
+    if (m2.w > 0.0f)
+        col = add(col, mul(scale(xyz(m2), m2.w), tp));
+    if (S->n_light == 0)
+        return false;
+    lidx = pcg_hash(seed) % S->n_light;
+    light_ray(S->light[2 * lidx], pos, L, dist2);
+    return true;
+}
+
+// Cook-Torrance contribution of a light that reaches the hit (Renderer.cu:351-360).
+__device__ __forceinline__ void add_light(CScene* S, int32_t mid, uint32_t lidx, f3 nrm, f3 rd, f3 L, f3 tp,
+                                          f3& col)
+{
+    const float4 m0 = S->mat[3 * mid + 0];
+    const float4 m1 = S->mat[3 * mid + 1];
+    const f3 albedo = xyz(m0);
+    const float metallic = m1.w;
+    const f3 base = add(scale(xyz(m1), 1.0f - metallic), scale(albedo, metallic));
+    const f3 f = cook_torrance(albedo, base, metallic, m0.w, nrm, neg(rd), L);
+    const float4 l0 = S->light[2 * lidx + 0];
+    const float4 l1 = S->light[2 * lidx + 1];
+    col = add(col, mul(mul(scale(xyz(l1), l0.w), f), tp));
+}
+
+// Russian roulette and the next direction (Renderer.cu:362-381).  Returns
+// true when the path ends here.
+__device__ __forceinline__ bool next_bounce(CScene* S, int32_t mid, f3 nrm, f3 so, f3& tp, f3& ro, f3& rd,
+                                            int32_t& bounce, uint32_t& seed)
+{
+    const float4 m0 = S->mat[3 * mid + 0];
+    const float4 m1 = S->mat[3 * mid + 1];
+    tp = mul(tp, xyz(m0));
+    ro = so;
+    const float p = gmax(0.1f, gmin(1.0f, length(tp)));
+    if (pcg_float(seed) > p)
+        return true;
+    tp = divs(tp, p);
+    rd = sample_next(nrm, m1.w > 0.0f, m0.w, seed);
+    ++bounce;
+    if (bounce >= S->max_bounces)
+        return true;
+    seed += (uint32_t)bounce;
+    return false;
+}
+
+// The path's sample (and the last frame's debug capture).
+__device__ __forceinline__ void store_sample(CScene* S, uint32_t frame, uint32_t px, f3 col)
+{
+    S->samples[(size_t)(frame - S->frame0) * (S->width * S->height) + px] = make_float4(col.x, col.y, col.z, 1.0f);
+    if (S->dbg_radiance && frame == S->frame0 + S->frames - 1u)
+        S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
 }
 
 template <uint32_t kWaves>
@@ -124,7 +220,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     // 64-bit counters at the end (and earlier if one could overflow); exact
     // retraces (rare) straight to their counter
     unsigned long long rays = 0;
-    uint32_t tests = 0, boxes = 0;
+    uint32_t tests = 0, boxes = 0, reused = 0;
 
     for (;;) {
         // ---------------- hand jobs to idle lanes (uniform control flow) ----------------
@@ -168,6 +264,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         }
                     }
                     state = kCont;
+                    if (S->first && !capture && S->max_bounces > 0) {
+                        // bounce 0 from the first-bounce cache: the primary hit, and
+                        // the chosen light's shadow decision when the scene has at
+                        // most kFirstMaxLights lights; the lane then enters the trace
+                        // loop at its first continuation ray, or takes another job
+                        const uint4 fh = S->first[px];
+                        Hit h;
+                        h.id = (int32_t)fh.x;
+                        h.t = __uint_as_float(fh.y);
+                        ++reused;
+                        bool done = false;
+                        if (h.id < 0) {
+                            if (S->sky_light)
+                                col = add(col, mul(mk(0.6f, 0.7f, 0.9f), tp));
+                            done = true;
+                        } else {
+                            bool bounce_end = true;
+                            if (shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2)) {
+                                if (S->n_light <= kFirstMaxLights) {
+                                    ++reused;
+                                    if ((fh.z >> lidx) & 1u)
+                                        add_light(S, mid, lidx, nrm, rd, L, tp, col);
+                                } else {
+                                    state = kShadow;
+                                    bounce_end = false;
+                                }
+                            }
+                            if (bounce_end)
+                                done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
+                        }
+                        if (done) {
+                            store_sample(S, frame, px, col);
+                            state = kIdle;
+                        }
+                    }
                 }
                 blk_next += nidle < avail ? nidle : avail;
                 continue;
@@ -246,76 +377,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                     if (S->sky_light)
                         col = add(col, mul(mk(0.6f, 0.7f, 0.9f), tp));
                     end_path = true;
+                } else if (shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2)) {
+                    state = kShadow;
                 } else {
-                    const f3 center = xyz(as_const(S->sph)[h.id]);
-                    f3 pos = add(sub(ro, center), scale(rd, h.t));
-                    nrm = normalize(pos);
-                    pos = add(pos, center);
-                    mid = S->sph_mat[h.id];
-                    const float4 m2 = S->mat[3 * mid + 2];
-                    if (m2.w > 0.0f)
-                        col = add(col, mul(scale(xyz(m2), m2.w), tp));
-                    so = add(pos, scale(nrm, 0.0001f));
-                    if (S->n_light > 0) {
-                        lidx = pcg_hash(seed) % S->n_light;
-                        L = sub(xyz(S->light[2 * lidx]), pos);
-                        dist2 = dot(L, L);
-                        L = normalize(L);
-                        state = kShadow;
-                    } else {
-                        end_bounce = true;
-                    }
+                    end_bounce = true;
                 }
             } else {
                 if (capture && S->dbg_shadow && (uint32_t)bounce < S->dbg_bounces)
                     S->dbg_shadow[(size_t)px * S->dbg_bounces + bounce] = h.id;
-                const float st = h.id < 0 ? -1.0f : h.t;
-                if (!(st > 0.0f && st * st < dist2)) {
-                    const float4 m0 = S->mat[3 * mid + 0];
-                    const float4 m1 = S->mat[3 * mid + 1];
-                    const f3 albedo = xyz(m0);
-                    const float metallic = m1.w;
-                    const f3 base = add(scale(xyz(m1), 1.0f - metallic), scale(albedo, metallic));
-                    const f3 f = cook_torrance(albedo, base, metallic, m0.w, nrm, neg(rd), L);
-                    const float4 l0 = S->light[2 * lidx + 0];
-                    const float4 l1 = S->light[2 * lidx + 1];
-                    col = add(col, mul(mul(scale(xyz(l1), l0.w), f), tp));
-                }
+                if (lit_by(h, dist2))
+                    add_light(S, mid, lidx, nrm, rd, L, tp, col);
                 end_bounce = true;
             }
         }
         if (end_bounce) {
-            const float4 m0 = S->mat[3 * mid + 0];
-            const float4 m1 = S->mat[3 * mid + 1];
-            tp = mul(tp, xyz(m0));
-            ro = so;
-            const float p = gmax(0.1f, gmin(1.0f, length(tp)));
-            if (pcg_float(seed) > p) {
+            if (next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed))
                 end_path = true;
-            } else {
-                tp = divs(tp, p);
-                rd = sample_next(nrm, m1.w > 0.0f, m0.w, seed);
-                ++bounce;
-                if (bounce >= S->max_bounces) {
-                    end_path = true;
-                } else {
-                    seed += (uint32_t)bounce;
-                    state = kCont;
-                }
-            }
+            else
+                state = kCont;
         }
-        if (__any(tests >= 0x80000000u || boxes >= 0x80000000u))
-            flush_counters(a.counters, lane, tests, boxes);
+        if (__any(tests >= 0x80000000u || boxes >= 0x80000000u || reused >= 0x80000000u))
+            flush_counters(a.counters, lane, tests, boxes, reused);
         if (end_path) {
-            S->samples[(size_t)(frame - S->frame0) * (S->width * S->height) + px] =
-                make_float4(col.x, col.y, col.z, 1.0f);
-            if (S->dbg_radiance && frame == S->frame0 + S->frames - 1u)
-                S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
+            store_sample(S, frame, px, col);
             state = kIdle;
         }
     }
 
-    flush_counters(a.counters, lane, tests, boxes);
+    flush_counters(a.counters, lane, tests, boxes, reused);
     if (lane == 0 && rays)
         atomicAdd(a.counters + kCtrRays, rays);
 }
@@ -385,6 +474,49 @@ __global__ __launch_bounds__(256) void k_camera_dirs(CamDirArgs c)
     c.dirs[px] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
+struct FirstArgs {
+    WfScene s;
+    uint4* first;
+};
+
+// One lane per pixel: the primary ray's closest hit and, for scenes of at
+// most kFirstMaxLights lights, the shadow decision towards every light, with
+// the path kernel's expressions (hit_frame, light_ray, lit_by) and queries,
+// so the cached results are the ones every frame's path would compute.
+__global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
+{
+    const WfScene& s = a.s;
+    const uint32_t px = blockIdx.x * 256u + threadIdx.x;
+    if (px >= s.width * s.height)
+        return;
+    const f3 ro = mk(s.cam_pos[0], s.cam_pos[1], s.cam_pos[2]);
+    const float4 dv = s.dirs[px];
+    const f3 rd = mk(dv.x, dv.y, dv.z);
+    const SphSrc flat{as_const(s.sph)};
+    auto trace = [&](f3 o, f3 d, float d2) -> Hit {
+        if (s.bvh.n_nodes) {
+            BvhQuery q;
+            return trace_bvh(s.bvh, flat, s.n_sph, o, d, d2, q);
+        }
+        uint32_t ex = 0;
+        return trace_filtered(flat, s.n_sph, o, d, ex);
+    };
+    const Hit h = trace(ro, rd, -1.0f);
+    uint32_t mask = 0u;
+    if (h.id >= 0 && s.n_light <= kFirstMaxLights) {
+        f3 pos, nrm, so;
+        hit_frame(as_const(s.sph)[h.id], h, ro, rd, pos, nrm, so);
+        for (uint32_t l = 0; l < s.n_light; ++l) {
+            f3 L;
+            float dist2;
+            light_ray(s.light[2 * l], pos, L, dist2);
+            if (lit_by(trace(so, L, dist2), dist2))
+                mask |= 1u << l;
+        }
+    }
+    a.first[px] = make_uint4((uint32_t)h.id, __float_as_uint(h.t), mask, 0u);
+}
+
 __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -426,6 +558,20 @@ hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream
     c.height = s.height;
     c.dirs = dirs;
     hipLaunchKernelGGL(k_camera_dirs, dim3((n + 255u) / 256u), dim3(256), 0, stream, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_first_hits(const WfScene& s, uint4* first, hipStream_t stream)
+{
+    const uint32_t n = s.width * s.height;
+    if (n == 0)
+        return hipSuccess;
+    if (!s.dirs || !first)
+        return hipErrorInvalidValue;
+    FirstArgs a;
+    a.s = s;
+    a.first = first;
+    hipLaunchKernelGGL(k_first_hit, dim3((n + 255u) / 256u), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -31,7 +31,11 @@ struct Renderer {
     bool has_scene = false;
     int kernel = ATX_KERNEL_WAVEFRONT;
     int accel = ATX_ACCEL_BVH;
-    DevBuf wf_table, wf_queue, wf_samples, wf_dirs;
+    DevBuf wf_table, wf_queue, wf_samples, wf_dirs, wf_first;
+    // wf_dirs / wf_first hold the primary rays and the first-bounce cache of
+    // this camera; set_scene and on_resize clear first_valid
+    atx_camera first_cam{};
+    bool first_valid = false;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kDefaultFramesPerLaunch;
@@ -142,7 +146,11 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             }
         }
         ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
-        for (WfScene& e : table) e.dirs = r->wf_dirs.as<float4>();
+        ATX_HIP(r->wf_first.reserve(npx * sizeof(uint4)));
+        for (WfScene& e : table) {
+            e.dirs = r->wf_dirs.as<float4>();
+            e.first = r->wf_first.as<uint4>();
+        }
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
         ATX_HIP(hipMemcpyAsync(r->wf_table.get(), table.data(), table.size() * sizeof(WfScene),
@@ -158,8 +166,14 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
-        // the camera is the same for every launch of this call
-        ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
+        // primary rays and the first-bounce cache: once per camera and scene
+        if (!r->first_valid || memcmp(&r->first_cam, cam, sizeof(*cam)) != 0) {
+            r->first_valid = false;
+            ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
+            ATX_HIP(launch_first_hits(table[0], r->wf_first.as<uint4>(), r->stream));
+            r->first_cam = *cam;
+            r->first_valid = true;
+        }
         for (size_t l = 0; l < table.size(); ++l) {
             WfPass wp{};
             wp.scenes = r->wf_table.as<WfScene>() + l;
@@ -205,7 +219,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         }
     }
     r->stats.samples = (uint64_t)npx * frames;
-    r->stats.rays = ctr[kCtrRays];
+    r->stats.rays = ctr[kCtrRays] + ctr[kCtrReused];
+    r->stats.reused_rays = ctr[kCtrReused];
     r->stats.sphere_tests = ctr[kCtrTests];
     r->stats.exact_retraces = ctr[kCtrExact];
     r->stats.box_tests = ctr[kCtrBoxes];
@@ -277,6 +292,7 @@ int atx_renderer_on_resize(atx_renderer* h, uint32_t width, uint32_t height)
     const size_t npx = (size_t)width * height;
     ATX_HIP(hipStreamSynchronize(r->stream));
     r->image.release(); r->accum.release(); r->comp.release();
+    r->first_valid = false;
     ATX_HIP(r->image.reserve(npx * sizeof(uint32_t)));
     ATX_HIP(r->accum.reserve(npx * sizeof(float4)));
     ATX_HIP(r->comp.reserve(npx * sizeof(float4)));
@@ -293,6 +309,7 @@ int atx_renderer_set_scene(atx_renderer* h, const atx_sphere* spheres, uint64_t 
     Renderer* r = reinterpret_cast<Renderer*>(h);
     ATX_CHECK_ARG(r, "renderer is NULL");
     ATX_HIP(hipStreamSynchronize(r->stream));
+    r->first_valid = false;
     int rc = upload_scene(spheres, num_spheres, materials, num_materials, lights, num_lights, &r->geo);
     if (rc != ATX_OK) return rc;
     r->has_scene = true;

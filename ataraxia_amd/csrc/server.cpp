@@ -52,6 +52,7 @@ struct Request {
     DevScene geo;
     DevBuf accum, comp, image;
     DevBuf dirs;                    // primary ray direction per pixel
+    DevBuf first;                   // first-bounce cache per pixel (launch_first_hits)
     uint32_t frames_done = 0;
     // results
     std::vector<uint32_t> rgba;
@@ -100,14 +101,14 @@ static double now_s(const Server* s)
 }
 
 // Device bytes a request holds while admitted: scene, tree, accumulation,
-// compensation, image, primary ray directions, and its share of the tick's
-// sample buffer.
+// compensation, image, primary ray directions, first-bounce cache, and its
+// share of the tick's sample buffer.
 static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_tick)
 {
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
-           npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16;
+           npx * (16 + (kahan ? 16 : 0) + 4 + 16 + 16) + npx * f * 16;
 }
 
 // A launch's paths must stay below 2^31 (the kernel's 32-bit job indices).
@@ -126,15 +127,17 @@ static bool admit(Server* s, Request& r)
                           r.lights.data(), r.lights.size(), &r.geo);
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     if (rc == ATX_OK && (r.accum.reserve(npx * 16) != hipSuccess || r.image.reserve(npx * 4) != hipSuccess ||
-                         r.dirs.reserve(npx * 16) != hipSuccess ||
+                         r.dirs.reserve(npx * 16) != hipSuccess || r.first.reserve(npx * 16) != hipSuccess ||
                          (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
         rc = ATX_ERR_NOMEM;
     if (rc == ATX_OK) {
         // on the worker's stream, ahead of the request's first launch
         WfScene w;
-        fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, ATX_ACCEL_NONE, &w);
-        if (launch_camera_dirs(w, r.dirs.as<float4>(), s->stream) != hipSuccess) {
-            set_error("primary ray directions: launch failed");
+        fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, (int)s->cfg.accel, &w);
+        w.dirs = r.dirs.as<float4>();
+        if (launch_camera_dirs(w, r.dirs.as<float4>(), s->stream) != hipSuccess ||
+            launch_first_hits(w, r.first.as<uint4>(), s->stream) != hipSuccess) {
+            set_error("primary rays: launch failed");
             rc = ATX_ERR_HIP;
         }
     }
@@ -145,6 +148,7 @@ static bool admit(Server* s, Request& r)
         r.comp.release();
         r.image.release();
         r.dirs.release();
+        r.first.release();
         return false;
     }
     return true;
@@ -159,6 +163,7 @@ static void release_device(Server* s, Request& r)
     r.comp.release();
     r.image.release();
     r.dirs.release();
+    r.first.release();
     std::lock_guard<std::mutex> lk(s->mu);
     s->stats.hbm_in_use -= r.device_bytes;
 }
@@ -280,6 +285,7 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
                       s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(), s->cfg.kahan != 0,
                       r->frames_done + 1u, std::min(F, r->spp - r->frames_done), (int)s->cfg.accel, &w);
         w.dirs = r->dirs.as<float4>();
+        w.first = r->first.as<uint4>();
         table.push_back(w);
     }
     WfPass wp{};

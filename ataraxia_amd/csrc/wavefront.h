@@ -57,6 +57,7 @@ struct WfScene {
     int32_t* dbg_shadow;
     float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
+    const uint4* first;    // first-bounce cache per pixel (launch_first_hits); nullptr: traced per path
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -90,6 +91,15 @@ hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
 // Writes the primary ray direction of every pixel of s's camera to dirs
 // (width * height float4, w = 0): camera_ray, computed once instead of per path.
 hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream);
+// The primary ray of a pixel has no jitter, so its closest hit, and the
+// shadow ray from that hit towards each light, are the same rays in every
+// frame.  Writes per pixel of s (s.dirs must hold the primary directions):
+// x = the primary ray's closest sphere (-1: miss), y = its t (float bits),
+// z = bit l set when light l lights the hit (the shadow decision of light l;
+// scenes of at most kFirstMaxLights lights), w = 0.  The path kernel then
+// starts each path at its first continuation ray.
+constexpr uint32_t kFirstMaxLights = 32;
+hipError_t launch_first_hits(const WfScene& s, uint4* first, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
 int wavefront_max_blocks_per_cu(bool flat_only);
 

@@ -146,6 +146,7 @@ def roofline(st, traffic):
             "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": kernel_ms,
             "launches": launches, "sphere_tests_per_launch": st["sphere_tests"] / launches,
             "box_tests_per_launch": st["box_tests"] / launches, "rays_per_launch": st["rays"] / launches,
+            "reused_rays_per_launch": st["reused_rays"] / launches,
             "exact_retraces": st["exact_retraces"]}
 
 

@@ -91,7 +91,8 @@ typedef struct atx_camera {
 /* Counters of the last render call. */
 typedef struct atx_render_stats {
     uint64_t samples;        /* path samples (pixels x frames)                 */
-    uint64_t rays;           /* rays traced: primary/continuation + shadow      */
+    uint64_t rays;           /* rays of the rendered paths: primary/continuation
+                                + shadow (as the reference traces them)         */
     uint64_t sphere_tests;   /* ray-sphere tests performed                      */
     uint64_t exact_retraces; /* traces resolved by the exact fallback loop      */
     uint64_t box_tests;      /* ray-box tests of BVH nodes                      */
@@ -99,6 +100,10 @@ typedef struct atx_render_stats {
     uint64_t launches;       /* path-kernel launches                            */
     double path_ms;          /* HIP-event time of the path kernel alone, summed
                                 over its launches (renderer; 0 for batches)     */
+    uint64_t reused_rays;    /* of `rays`, those answered from the per-pixel
+                                first-bounce cache (a primary ray and its shadow
+                                ray are the same rays in every frame); the rest
+                                were traced                                     */
 } atx_render_stats;
 
 const char* atx_last_error(void);

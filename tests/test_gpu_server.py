@@ -52,10 +52,10 @@ def test_server_staggered_arrivals_and_budget():
     scenes = mixed_scenes()
     ref, _ = serve_all(scenes, frames_per_tick=1)
     # admit roughly one request at a time: budget just above the largest request
-    # (accumulation + image 20 B/px, primary ray directions 16 B/px, one frame
-    # of samples 16 B/px, scene and tree)
+    # (accumulation + image 20 B/px, primary ray directions 16 B/px, first-bounce
+    # cache 16 B/px, one frame of samples 16 B/px, scene and tree)
     npx = max(s.camera.width * s.camera.height for s in scenes)
-    budget = npx * 52 + 4096 * 64 + (1 << 16)
+    budget = npx * 68 + 4096 * 64 + (1 << 16)
     res, st = serve_all(scenes, delays=[0.0, 0.01, 0.0, 0.02, 0.0, 0.01], frames_per_tick=1,
                         hbm_budget_bytes=budget)
     assert st["hbm_high_water"] <= budget
