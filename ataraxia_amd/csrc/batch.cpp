@@ -138,18 +138,19 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             fill_wf_scene(e.geo, e.settings, e.cam, e.accum.as<float4>(),
                           b->kahan ? e.comp.as<float4>() : nullptr, e.image.as<uint32_t>(), b->kahan,
                           f0 + 1u, std::min(F, spp - f0), b->accel, &w);
-            w.dirs = e.dirs.as<float4>();
-            w.first = e.first.as<uint4>();
+            // the first-bounce cache pays from two frames per scene on
+            const bool cache = e.spp >= 2;
             if (!e.dirs_ready) {
                 const size_t npx = (size_t)e.cam.width * e.cam.height;
                 ATX_HIP(e.dirs.reserve(npx * sizeof(float4)));
-                ATX_HIP(e.first.reserve(npx * sizeof(uint4)));
+                if (cache) ATX_HIP(e.first.reserve(npx * sizeof(uint4)));
                 w.dirs = e.dirs.as<float4>();
-                w.first = e.first.as<uint4>();
                 ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
-                ATX_HIP(launch_first_hits(w, e.first.as<uint4>(), b->stream));
+                if (cache) ATX_HIP(launch_first_hits(w, e.first.as<uint4>(), b->stream));
                 e.dirs_ready = true;
             }
+            w.dirs = e.dirs.as<float4>();
+            w.first = cache ? e.first.as<uint4>() : nullptr;
             if (e.capture) {
                 w.dbg_radiance = e.dbg_rad.as<float4>();
                 w.dbg_hit = e.dbg_hit.as<int32_t>();
@@ -167,8 +168,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
         const WfScene* lt = table.data() + b->launch_off[l];
-        const uint32_t P = b->chunk_px ? b->chunk_px
-                                       : auto_chunk_px(launch_samples(lt, b->launch_scenes[l]), b->grid, b->flat_only);
+        const uint32_t P = b->chunk_px ? b->chunk_px : auto_chunk_px(launch_samples(lt, b->launch_scenes[l]), b->grid);
         WfPass lp{};
         if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], P,
                            b->d_samples.as<float4>(), &lp)) {

@@ -32,10 +32,10 @@ struct Renderer {
     int kernel = ATX_KERNEL_WAVEFRONT;
     int accel = ATX_ACCEL_BVH;
     DevBuf wf_table, wf_queue, wf_samples, wf_dirs, wf_first;
-    // wf_dirs / wf_first hold the primary rays and the first-bounce cache of
-    // this camera; set_scene and on_resize clear first_valid
-    atx_camera first_cam{};
-    bool first_valid = false;
+    // wf_dirs holds the primary rays of dirs_cam; wf_first the first-bounce
+    // cache of that camera and the current scene (set_scene clears first_valid)
+    atx_camera dirs_cam{}, prev_cam{};
+    bool dirs_valid = false, first_valid = false, prev_valid = false;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kDefaultFramesPerLaunch;
@@ -110,6 +110,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     std::vector<WfScene> table;
     std::vector<uint32_t> launch_frames, launch_chunk;
     const bool wf = r->kernel == ATX_KERNEL_WAVEFRONT && frames > 0;
+    bool use_first = false;
     WfLaunch shape{0, false};
     if (wf) {
         uint32_t fi = r->frame_index, left = frames;
@@ -136,8 +137,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         shape.flat_only = wf_flat_only(table.data(), table.size());
         shape.workgroups = wavefront_full_grid(shape.flat_only);
         for (WfScene& e : table) {
-            const uint32_t P = r->wf_chunk ? r->wf_chunk
-                                           : auto_chunk_px(launch_samples(&e, 1), shape.workgroups, shape.flat_only);
+            const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(launch_samples(&e, 1), shape.workgroups);
             launch_chunk.push_back(P);
             WfPass lp{};
             if (!layout_launch(&e, 1, P, r->wf_samples.as<float4>(), &lp)) {
@@ -145,11 +145,17 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
                 return ATX_ERR_INVALID;
             }
         }
+        // The first-bounce cache pays from two frames per camera on: a call
+        // of one frame uses it once the camera has stayed the same since the
+        // previous call (a camera that moves every frame never builds it).
+        use_first = frames >= 2 || (r->prev_valid && memcmp(&r->prev_cam, cam, sizeof(*cam)) == 0);
+        r->prev_cam = *cam;
+        r->prev_valid = true;
         ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
-        ATX_HIP(r->wf_first.reserve(npx * sizeof(uint4)));
+        if (use_first) ATX_HIP(r->wf_first.reserve(npx * sizeof(uint4)));
         for (WfScene& e : table) {
             e.dirs = r->wf_dirs.as<float4>();
-            e.first = r->wf_first.as<uint4>();
+            e.first = use_first ? r->wf_first.as<uint4>() : nullptr;
         }
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
@@ -166,12 +172,15 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
-        // primary rays and the first-bounce cache: once per camera and scene
-        if (!r->first_valid || memcmp(&r->first_cam, cam, sizeof(*cam)) != 0) {
-            r->first_valid = false;
+        // primary rays once per camera, the first-bounce cache once per camera and scene
+        if (!r->dirs_valid || memcmp(&r->dirs_cam, cam, sizeof(*cam)) != 0) {
+            r->dirs_valid = r->first_valid = false;
             ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
+            r->dirs_cam = *cam;
+            r->dirs_valid = true;
+        }
+        if (use_first && !r->first_valid) {
             ATX_HIP(launch_first_hits(table[0], r->wf_first.as<uint4>(), r->stream));
-            r->first_cam = *cam;
             r->first_valid = true;
         }
         for (size_t l = 0; l < table.size(); ++l) {
@@ -292,7 +301,7 @@ int atx_renderer_on_resize(atx_renderer* h, uint32_t width, uint32_t height)
     const size_t npx = (size_t)width * height;
     ATX_HIP(hipStreamSynchronize(r->stream));
     r->image.release(); r->accum.release(); r->comp.release();
-    r->first_valid = false;
+    r->dirs_valid = r->first_valid = r->prev_valid = false;
     ATX_HIP(r->image.reserve(npx * sizeof(uint32_t)));
     ATX_HIP(r->accum.reserve(npx * sizeof(float4)));
     ATX_HIP(r->comp.reserve(npx * sizeof(float4)));
@@ -309,7 +318,7 @@ int atx_renderer_set_scene(atx_renderer* h, const atx_sphere* spheres, uint64_t 
     Renderer* r = reinterpret_cast<Renderer*>(h);
     ATX_CHECK_ARG(r, "renderer is NULL");
     ATX_HIP(hipStreamSynchronize(r->stream));
-    r->first_valid = false;
+    r->first_valid = r->prev_valid = false;
     int rc = upload_scene(spheres, num_spheres, materials, num_materials, lights, num_lights, &r->geo);
     if (rc != ATX_OK) return rc;
     r->has_scene = true;

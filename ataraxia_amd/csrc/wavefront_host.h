@@ -109,21 +109,15 @@ inline bool wf_flat_only(const WfScene* tab, size_t n)
 // Jobs per chunk for a launch of `jobs` jobs on a persistent grid of
 // `workgroups` (4 waves each) when the caller gives none.  Small chunks share
 // the launch out evenly to the end; large ones keep a wave on neighbouring
-// pixels and fetch fewer blocks.  Measured on the 1080p workload (1 to 64
-// frames per launch, chunks of 64 to 512 jobs): launches that traverse trees
-// are fastest with about ten chunks per wave (the largest of 512, 256, 128
-// jobs that leaves that many; within 3 % of the best at every frame count),
-// flat-loop launches with chunks of 128 jobs, or of 64 when a launch has
-// fewer than four chunks of 128 per wave (one 1080p frame).
-inline uint32_t auto_chunk_px(uint64_t jobs, uint32_t workgroups, bool flat_only)
+// pixels (and in a many-frame launch on the same pixel) and fetch fewer
+// blocks.  Measured on the 1080p workload (1 to 64 frames per launch, chunks
+// of 64 to 1024 jobs, both kernel instances): 512 jobs from about 5000 jobs
+// per wave on (16 frames and more), 256 below; within 6 % of the best chunk
+// at every frame count.
+inline uint32_t auto_chunk_px(uint64_t jobs, uint32_t workgroups)
 {
     const uint64_t per_wave = jobs / (4ull * (workgroups ? workgroups : 1u));
-    if (flat_only)
-        return per_wave >= 4ull * 128u ? 128u : 64u;
-    uint32_t c = 512u;
-    while (c > 128u && per_wave < 10ull * c)
-        c >>= 1;
-    return c;
+    return per_wave >= 5000u ? 512u : 256u;
 }
 
 constexpr uint32_t kDefaultFramesPerLaunch = 64;

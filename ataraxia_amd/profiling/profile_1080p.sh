@@ -15,6 +15,7 @@ REPO=$(pwd)
 KERNEL=${KERNEL:-wavefront}
 ACCEL=${ACCEL:-bvh}
 OUT=${OUT:-$REPO/profile_out/prof_${KERNEL}_$ACCEL}
+case "$OUT" in /*) ;; *) OUT="$REPO/$OUT" ;; esac   # the passes run from /tmp
 STEPS=${STEPS:-128}
 if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-64}; fi
 BENCH=("$REPO/bench.py" --kernel "$KERNEL" --accel "$ACCEL" --no-serve --no-cpu-baseline)

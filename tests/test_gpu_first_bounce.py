@@ -4,7 +4,10 @@ A pixel's primary ray has no jitter, so it is the same ray in every frame,
 and so is the shadow ray from its hit towards each light.  The persistent
 kernel takes bounce 0 of every path from a per-pixel cache (closest primary
 hit + one shadow-decision bit per light, scenes of at most 32 lights) that
-k_first_hit computes once per camera and scene.  Every case here renders
+k_first_hit computes once per camera and scene (the renderer builds it for a
+call of two or more frames, or once the camera has not moved since the
+previous call; batches and the server for scenes of two or more frames).
+Every case here renders
 through that cache (not the debug capture, which traces every ray) and
 compares with the plain kernel, which traces every ray as the reference's
 loop does, and with the oracle: accumulation and image bit for bit, the same
@@ -99,12 +102,37 @@ def test_cached_bounce0_equals_plain_and_oracle(name):
 
 def test_one_frame_calls_reuse_the_cache():
     """render_frames(cam, 1) four times (the reference's Render loop) equals
-    four frames in one call; the cache is built on the first call only."""
+    four frames in one call; the first call of a camera traces every ray,
+    the next ones (same camera) use the cache."""
     sc = CASES["plain-cpu"]
     acc1, img1, rays1, reused1 = _render(sc, ATX_KERNEL_WAVEFRONT, 4, 1)
     acc4, img4, rays4, reused4 = _render(sc, ATX_KERNEL_WAVEFRONT, 4, 4)
     assert np.array_equal(acc1, acc4) and np.array_equal(img1, img4)
-    assert rays1 == rays4 and reused1 == reused4 > 0
+    assert rays1 == rays4 and reused4 > 0 and reused1 * 4 == reused4 * 3
+
+
+def test_moving_camera_never_builds_the_cache():
+    """One frame per call with a different camera each call: no call reuses
+    a ray, and the images equal the plain kernel's."""
+    sc = CASES["plain-cpu"]
+    cams = [Camera.looking_at((x, 4.0, 14.0), width=sc.camera.width, height=sc.camera.height)
+            for x in (0.0, 0.5, 1.0, 0.5)]
+    out = {}
+    for kernel in (ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT):
+        with Renderer() as r:
+            r.set_kernel(kernel)
+            r.on_resize(sc.camera.width, sc.camera.height)
+            r.set_settings(sc.settings)
+            r.set_scene(sc)
+            accs = []
+            for cam in cams:
+                r.reset_frame_index()
+                r.render_frames(cam, 1)
+                assert r.stats()["reused_rays"] == 0
+                accs.append(r.get_accumulation().copy())
+        out[kernel] = accs
+    for x, y in zip(out[ATX_KERNEL_PLAIN], out[ATX_KERNEL_WAVEFRONT]):
+        assert np.array_equal(x, y)
 
 
 def test_cache_follows_camera_and_scene():
@@ -129,7 +157,8 @@ def test_cache_follows_camera_and_scene():
             r.set_scene(other)
             r.reset_frame_index()
             r.render_frames(cam_b, 1)
-            r.render_frames(cam_b, 1)
+            r.render_frames(cam_b, 1)     # same camera as the previous call: uses a cache of the new scene
+            assert (r.stats()["reused_rays"] > 0) == (kernel == ATX_KERNEL_WAVEFRONT)
             c = r.get_accumulation().copy()
         out[kernel] = (a, b, c)
     for x, y in zip(out[ATX_KERNEL_PLAIN], out[ATX_KERNEL_WAVEFRONT]):

@@ -69,20 +69,17 @@ int main()
     // chunk size chosen per launch: 1080p frames on a 256-CU grid
     // (tree launches 6 workgroups per CU, flat-loop launches 7)
     const uint64_t f1080 = 1920ull * 1080;
-    CHECK(auto_chunk_px(f1080, 1536, false) == 128);        // 337 jobs per wave
-    CHECK(auto_chunk_px(2 * f1080, 1536, false) == 128);
-    CHECK(auto_chunk_px(8 * f1080, 1536, false) == 256);    // 2700 per wave
-    CHECK(auto_chunk_px(32 * f1080, 1536, false) == 512);
-    CHECK(auto_chunk_px(64 * f1080, 1536, false) == 512);
-    CHECK(auto_chunk_px(f1080, 1792, true) == 64);          // 289 per wave
-    CHECK(auto_chunk_px(2 * f1080, 1792, true) == 128);
-    CHECK(auto_chunk_px(64 * f1080, 1792, true) == 128);
-    CHECK(auto_chunk_px(1, 0, false) == 128 && auto_chunk_px(0, 1, true) == 64);
-    for (uint64_t jobs : {uint64_t(1), uint64_t(1000), f1080, 64 * f1080, uint64_t(1) << 31})
-        for (bool flat : {false, true}) {
-            const uint32_t c = auto_chunk_px(jobs, 1536, flat);
-            CHECK(c % 64 == 0 && c >= 64 && c <= kMaxChunkPx);
-        }
+    CHECK(auto_chunk_px(f1080, 1536) == 256);          // 337 jobs per wave
+    CHECK(auto_chunk_px(8 * f1080, 1536) == 256);      // 2700
+    CHECK(auto_chunk_px(16 * f1080, 1536) == 512);     // 5400
+    CHECK(auto_chunk_px(64 * f1080, 1536) == 512);
+    CHECK(auto_chunk_px(16 * f1080, 1792) == 256);     // 4628
+    CHECK(auto_chunk_px(32 * f1080, 1792) == 512);
+    CHECK(auto_chunk_px(1, 0) == 256 && auto_chunk_px(0, 1) == 256);
+    for (uint64_t jobs : {uint64_t(1), uint64_t(1000), f1080, 64 * f1080, uint64_t(1) << 31}) {
+        const uint32_t c = auto_chunk_px(jobs, 1536);
+        CHECK(c % 64 == 0 && c >= 64 && c <= kMaxChunkPx);
+    }
 
     if (failures == 0) std::printf("wavefront layout: all checks passed\n");
     return failures == 0 ? 0 : 1;
