@@ -140,17 +140,18 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
                           f0 + 1u, std::min(F, spp - f0), b->accel, &w);
             // the first-bounce cache pays from two frames per scene on
             const bool cache = e.spp >= 2;
+            const size_t npx = (size_t)e.cam.width * e.cam.height;
             if (!e.dirs_ready) {
-                const size_t npx = (size_t)e.cam.width * e.cam.height;
                 ATX_HIP(e.dirs.reserve(npx * sizeof(float4)));
-                if (cache) ATX_HIP(e.first.reserve(npx * sizeof(uint4)));
-                w.dirs = e.dirs.as<float4>();
-                ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
-                if (cache) ATX_HIP(launch_first_hits(w, e.first.as<uint4>(), b->stream));
-                e.dirs_ready = true;
+                if (cache) ATX_HIP(e.first.reserve(first_cache_bytes(npx, w.n_light)));
             }
             w.dirs = e.dirs.as<float4>();
-            w.first = cache ? e.first.as<uint4>() : nullptr;
+            if (cache) set_first_cache(w, e.first.get());
+            if (!e.dirs_ready) {
+                ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
+                if (cache) ATX_HIP(launch_first_hits(w, b->stream));
+                e.dirs_ready = true;
+            }
             if (e.capture) {
                 w.dbg_radiance = e.dbg_rad.as<float4>();
                 w.dbg_hit = e.dbg_hit.as<int32_t>();

@@ -129,6 +129,14 @@ __device__ __forceinline__ bool lit_by(const Hit& h, float dist2)
     return !(st > 0.0f && st * st < dist2);
 }
 
+// The hit material's emission, added to col (Renderer.cu:334-337).
+__device__ __forceinline__ void add_emission(const float4* mat, int32_t mid, f3 tp, f3& col)
+{
+    const float4 m2 = mat[3 * mid + 2];
+    if (m2.w > 0.0f)
+        col = add(col, mul(scale(xyz(m2), m2.w), tp));
+}
+
 // Emission of a continuation hit and the light its shadow ray goes to.
 // Returns true when a shadow ray follows (the scene has lights).
 __device__ __forceinline__ bool shade_hit(CScene* S, const Hit& h, f3 ro, f3 rd, f3 tp, uint32_t seed, f3& col,
@@ -137,9 +145,7 @@ __device__ __forceinline__ bool shade_hit(CScene* S, const Hit& h, f3 ro, f3 rd,
     f3 pos;
     hit_frame(as_const(S->sph)[h.id], h, ro, rd, pos, nrm, so);
     mid = S->sph_mat[h.id];
-    const float4 m2 = S->mat[3 * mid + 2];
-    if (m2.w > 0.0f)
-        col = add(col, mul(scale(xyz(m2), m2.w), tp));
+    add_emission(S->mat, mid, tp, col);
     if (S->n_light == 0)
         return false;
     lidx = pcg_hash(seed) % S->n_light;
@@ -147,19 +153,26 @@ __device__ __forceinline__ bool shade_hit(CScene* S, const Hit& h, f3 ro, f3 rd,
     return true;
 }
 
-// Cook-Torrance contribution of a light that reaches the hit (Renderer.cu:351-360).
-__device__ __forceinline__ void add_light(CScene* S, int32_t mid, uint32_t lidx, f3 nrm, f3 rd, f3 L, f3 tp,
-                                          f3& col)
+// Cook-Torrance term of a light that reaches the hit, weighted by the path
+// throughput (Renderer.cu:351-360).
+__device__ __forceinline__ f3 light_term(const float4* mat, const float4* light, int32_t mid, uint32_t lidx, f3 nrm,
+                                         f3 rd, f3 L, f3 tp)
 {
-    const float4 m0 = S->mat[3 * mid + 0];
-    const float4 m1 = S->mat[3 * mid + 1];
+    const float4 m0 = mat[3 * mid + 0];
+    const float4 m1 = mat[3 * mid + 1];
     const f3 albedo = xyz(m0);
     const float metallic = m1.w;
     const f3 base = add(scale(xyz(m1), 1.0f - metallic), scale(albedo, metallic));
     const f3 f = cook_torrance(albedo, base, metallic, m0.w, nrm, neg(rd), L);
-    const float4 l0 = S->light[2 * lidx + 0];
-    const float4 l1 = S->light[2 * lidx + 1];
-    col = add(col, mul(mul(scale(xyz(l1), l0.w), f), tp));
+    const float4 l0 = light[2 * lidx + 0];
+    const float4 l1 = light[2 * lidx + 1];
+    return mul(mul(scale(xyz(l1), l0.w), f), tp);
+}
+
+__device__ __forceinline__ void add_light(CScene* S, int32_t mid, uint32_t lidx, f3 nrm, f3 rd, f3 L, f3 tp,
+                                          f3& col)
+{
+    col = add(col, light_term(S->mat, S->light, mid, lidx, nrm, rd, L, tp));
 }
 
 // Russian roulette and the next direction (Renderer.cu:362-381).  Returns
@@ -265,34 +278,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                     }
                     state = kCont;
                     if (S->first && !capture && S->max_bounces > 0) {
-                        // bounce 0 from the first-bounce cache: the primary hit, and
-                        // the chosen light's shadow decision when the scene has at
-                        // most kFirstMaxLights lights; the lane then enters the trace
-                        // loop at its first continuation ray, or takes another job
-                        const uint4 fh = S->first[px];
-                        Hit h;
-                        h.id = (int32_t)fh.x;
-                        h.t = __uint_as_float(fh.y);
+                        // bounce 0 from the first-bounce cache; the lane then enters
+                        // the trace loop at its first continuation ray, or ends the
+                        // path and takes another job
+                        // the frame's light is known before the record arrives: issue
+                        // the record's four loads and the light term's load together
+                        const uint32_t nl = S->n_light;
+                        const uint32_t lsel = nl > 0u ? pcg_hash(seed) % nl : 0u;
+                        const uint4* fr = S->first + 4 * (size_t)px;
+                        const uint4 r0 = fr[0], r1 = fr[1], r2 = fr[2], r3 = fr[3];
+                        const float4 lt = S->first_light ? S->first_light[(size_t)px * nl + lsel]
+                                                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        const int32_t hid = (int32_t)r0.x;
                         ++reused;
                         bool done = false;
-                        if (h.id < 0) {
+                        if (hid < 0) {
                             if (S->sky_light)
                                 col = add(col, mul(mk(0.6f, 0.7f, 0.9f), tp));
                             done = true;
+                        } else if (S->n_light > kFirstMaxLights) {
+                            // too many lights for the mask: shade the cached hit and
+                            // trace the shadow ray
+                            Hit h;
+                            h.id = hid;
+                            h.t = __uint_as_float(r0.y);
+                            shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2);
+                            state = kShadow;
                         } else {
-                            bool bounce_end = true;
-                            if (shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2)) {
-                                if (S->n_light <= kFirstMaxLights) {
-                                    ++reused;
-                                    if ((fh.z >> lidx) & 1u)
-                                        add_light(S, mid, lidx, nrm, rd, L, tp, col);
-                                } else {
-                                    state = kShadow;
-                                    bounce_end = false;
-                                }
+                            nrm = mk(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
+                            so = mk(__uint_as_float(r1.w), __uint_as_float(r2.x), __uint_as_float(r2.y));
+                            col = mk(__uint_as_float(r2.z), __uint_as_float(r2.w), __uint_as_float(r3.x));
+                            mid = (int32_t)r0.w;
+                            if (nl > 0u) {
+                                // the frame's light, and its term if it reaches the hit
+                                lidx = lsel;
+                                ++reused;
+                                if ((r0.z >> lidx) & 1u)
+                                    col = add(col, xyz(lt));
                             }
-                            if (bounce_end)
-                                done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
+                            done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
                         }
                         if (done) {
                             store_sample(S, frame, px, col);
@@ -476,13 +500,15 @@ __global__ __launch_bounds__(256) void k_camera_dirs(CamDirArgs c)
 
 struct FirstArgs {
     WfScene s;
-    uint4* first;
+    uint4* rec;      // 4 per pixel
+    float4* light;   // n_light per pixel, or nullptr
 };
 
-// One lane per pixel: the primary ray's closest hit and, for scenes of at
-// most kFirstMaxLights lights, the shadow decision towards every light, with
-// the path kernel's expressions (hit_frame, light_ray, lit_by) and queries,
-// so the cached results are the ones every frame's path would compute.
+// One lane per pixel: the primary ray's closest hit, the shading at it, and
+// for scenes of at most kFirstMaxLights lights each light's shadow decision
+// and Cook-Torrance term, with the path kernel's expressions (hit_frame,
+// add_emission, light_ray, lit_by, light_term; tp = 1 at bounce 0) and
+// queries, so the cached values are the ones every frame's path computes.
 __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
 {
     const WfScene& s = a.s;
@@ -502,19 +528,38 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
         return trace_filtered(flat, s.n_sph, o, d, ex);
     };
     const Hit h = trace(ro, rd, -1.0f);
+    const f3 one = mk(1.0f, 1.0f, 1.0f);
+    const f3 zero = mk(0.0f, 0.0f, 0.0f);
+    f3 nrm = zero, so = zero, col = zero;
+    int32_t mid = 0;
     uint32_t mask = 0u;
-    if (h.id >= 0 && s.n_light <= kFirstMaxLights) {
-        f3 pos, nrm, so;
+    if (h.id >= 0) {
+        f3 pos;
         hit_frame(as_const(s.sph)[h.id], h, ro, rd, pos, nrm, so);
-        for (uint32_t l = 0; l < s.n_light; ++l) {
-            f3 L;
-            float dist2;
-            light_ray(s.light[2 * l], pos, L, dist2);
-            if (lit_by(trace(so, L, dist2), dist2))
-                mask |= 1u << l;
+        mid = s.sph_mat[h.id];
+        add_emission(s.mat, mid, one, col);
+        if (a.light) {
+            for (uint32_t l = 0; l < s.n_light; ++l) {
+                f3 L;
+                float dist2;
+                light_ray(s.light[2 * l], pos, L, dist2);
+                f3 term = zero;
+                if (lit_by(trace(so, L, dist2), dist2)) {
+                    mask |= 1u << l;
+                    term = light_term(s.mat, s.light, mid, l, nrm, rd, L, one);
+                }
+                a.light[(size_t)px * s.n_light + l] = make_float4(term.x, term.y, term.z, 0.0f);
+            }
         }
+    } else if (a.light) {
+        for (uint32_t l = 0; l < s.n_light; ++l)
+            a.light[(size_t)px * s.n_light + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    a.first[px] = make_uint4((uint32_t)h.id, __float_as_uint(h.t), mask, 0u);
+    uint4* r = a.rec + 4 * (size_t)px;
+    r[0] = make_uint4((uint32_t)h.id, __float_as_uint(h.t), mask, (uint32_t)mid);
+    r[1] = make_uint4(__float_as_uint(nrm.x), __float_as_uint(nrm.y), __float_as_uint(nrm.z), __float_as_uint(so.x));
+    r[2] = make_uint4(__float_as_uint(so.y), __float_as_uint(so.z), __float_as_uint(col.x), __float_as_uint(col.y));
+    r[3] = make_uint4(__float_as_uint(col.z), 0u, 0u, 0u);
 }
 
 __global__ void k_zero_u32(unsigned int* p, uint32_t n)
@@ -561,16 +606,18 @@ hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream
     return hipGetLastError();
 }
 
-hipError_t launch_first_hits(const WfScene& s, uint4* first, hipStream_t stream)
+hipError_t launch_first_hits(const WfScene& s, hipStream_t stream)
 {
     const uint32_t n = s.width * s.height;
     if (n == 0)
         return hipSuccess;
-    if (!s.dirs || !first)
+    const bool lights = s.n_light >= 1 && s.n_light <= kFirstMaxLights;
+    if (!s.dirs || !s.first || lights != (s.first_light != nullptr))
         return hipErrorInvalidValue;
     FirstArgs a;
     a.s = s;
-    a.first = first;
+    a.rec = const_cast<uint4*>(s.first);
+    a.light = const_cast<float4*>(s.first_light);
     hipLaunchKernelGGL(k_first_hit, dim3((n + 255u) / 256u), dim3(256), 0, stream, a);
     return hipGetLastError();
 }

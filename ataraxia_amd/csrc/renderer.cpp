@@ -152,10 +152,10 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         r->prev_cam = *cam;
         r->prev_valid = true;
         ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
-        if (use_first) ATX_HIP(r->wf_first.reserve(npx * sizeof(uint4)));
+        if (use_first) ATX_HIP(r->wf_first.reserve(first_cache_bytes(npx, r->geo.n_light)));
         for (WfScene& e : table) {
             e.dirs = r->wf_dirs.as<float4>();
-            e.first = use_first ? r->wf_first.as<uint4>() : nullptr;
+            if (use_first) set_first_cache(e, r->wf_first.get());
         }
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
@@ -180,7 +180,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             r->dirs_valid = true;
         }
         if (use_first && !r->first_valid) {
-            ATX_HIP(launch_first_hits(table[0], r->wf_first.as<uint4>(), r->stream));
+            ATX_HIP(launch_first_hits(table[0], r->stream));
             r->first_valid = true;
         }
         for (size_t l = 0; l < table.size(); ++l) {

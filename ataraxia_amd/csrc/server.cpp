@@ -108,7 +108,8 @@ static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_ti
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
-           npx * (16 + (kahan ? 16 : 0) + 4 + 16 + (r.spp >= 2 ? 16 : 0)) + npx * f * 16;
+           npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16 +
+           (r.spp >= 2 ? first_cache_bytes(npx, (uint32_t)r.lights.size()) : 0);
 }
 
 // A launch's paths must stay below 2^31 (the kernel's 32-bit job indices).
@@ -128,7 +129,7 @@ static bool admit(Server* s, Request& r)
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     if (rc == ATX_OK && (r.accum.reserve(npx * 16) != hipSuccess || r.image.reserve(npx * 4) != hipSuccess ||
                          r.dirs.reserve(npx * 16) != hipSuccess ||
-                         (r.spp >= 2 && r.first.reserve(npx * 16) != hipSuccess) ||
+                         (r.spp >= 2 && r.first.reserve(first_cache_bytes(npx, (uint32_t)r.lights.size())) != hipSuccess) ||
                          (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
         rc = ATX_ERR_NOMEM;
     if (rc == ATX_OK) {
@@ -136,9 +137,10 @@ static bool admit(Server* s, Request& r)
         WfScene w;
         fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, (int)s->cfg.accel, &w);
         w.dirs = r.dirs.as<float4>();
+        if (r.spp >= 2) set_first_cache(w, r.first.get());
         // the first-bounce cache pays from two frames per request on
         if (launch_camera_dirs(w, r.dirs.as<float4>(), s->stream) != hipSuccess ||
-            (r.spp >= 2 && launch_first_hits(w, r.first.as<uint4>(), s->stream) != hipSuccess)) {
+            (r.spp >= 2 && launch_first_hits(w, s->stream) != hipSuccess)) {
             set_error("primary rays: launch failed");
             rc = ATX_ERR_HIP;
         }
@@ -287,7 +289,7 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
                       s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(), s->cfg.kahan != 0,
                       r->frames_done + 1u, std::min(F, r->spp - r->frames_done), (int)s->cfg.accel, &w);
         w.dirs = r->dirs.as<float4>();
-        w.first = r->spp >= 2 ? r->first.as<uint4>() : nullptr;
+        if (r->spp >= 2) set_first_cache(w, r->first.get());
         table.push_back(w);
     }
     WfPass wp{};

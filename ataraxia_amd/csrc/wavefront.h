@@ -57,7 +57,8 @@ struct WfScene {
     int32_t* dbg_shadow;
     float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
-    const uint4* first;    // first-bounce cache per pixel (launch_first_hits); nullptr: traced per path
+    const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
+    const float4* first_light; // first-bounce light terms, n_light per pixel (nullptr: none)
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -91,15 +92,38 @@ hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
 // Writes the primary ray direction of every pixel of s's camera to dirs
 // (width * height float4, w = 0): camera_ray, computed once instead of per path.
 hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream);
-// The primary ray of a pixel has no jitter, so its closest hit, and the
-// shadow ray from that hit towards each light, are the same rays in every
-// frame.  Writes per pixel of s (s.dirs must hold the primary directions):
-// x = the primary ray's closest sphere (-1: miss), y = its t (float bits),
-// z = bit l set when light l lights the hit (the shadow decision of light l;
-// scenes of at most kFirstMaxLights lights), w = 0.  The path kernel then
-// starts each path at its first continuation ray.
+// First-bounce cache.  The primary ray of a pixel has no jitter, so its
+// closest hit, the shading at that hit and the shadow ray from it towards
+// each light are the same in every frame; only the light a frame picks and
+// the direction it continues in are random.  One allocation of
+// first_cache_bytes(pixels, lights) per camera and scene holds
+//  * per pixel a 64-byte record (4 x uint4, floats as bits):
+//      [0] closest sphere of the primary ray (-1: miss), its t, the lit mask
+//          (bit l: light l reaches the hit; scenes of at most kFirstMaxLights
+//          lights), the hit's material;
+//      [1] normal x, y, z, origin of the rays leaving the hit x;
+//      [2] origin y, z, radiance after the hit's emission r, g;
+//      [3] radiance b, 0, 0, 0;
+//  * then, for scenes of 1..kFirstMaxLights lights, per pixel and light the
+//    light's Cook-Torrance term at the hit (float4, w = 0; zero when the
+//    light does not reach the hit).
+// The path kernel then does bounce 0 from it where it hands out a job.
 constexpr uint32_t kFirstMaxLights = 32;
-hipError_t launch_first_hits(const WfScene& s, uint4* first, hipStream_t stream);
+inline size_t first_cache_bytes(size_t pixels, uint32_t lights)
+{
+    return pixels * 64 + (lights >= 1 && lights <= kFirstMaxLights ? pixels * lights * 16 : 0);
+}
+// Points s.first / s.first_light into buf (first_cache_bytes of s's pixels and lights).
+inline void set_first_cache(WfScene& s, void* buf)
+{
+    const size_t npx = (size_t)s.width * s.height;
+    s.first = (const uint4*)buf;
+    s.first_light = (s.n_light >= 1 && s.n_light <= kFirstMaxLights) ? (const float4*)((char*)buf + npx * 64)
+                                                                      : nullptr;
+}
+// Writes the cache s.first / s.first_light point to (s.dirs must hold the
+// primary directions).
+hipError_t launch_first_hits(const WfScene& s, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
 int wavefront_max_blocks_per_cu(bool flat_only);
 
