@@ -19,15 +19,16 @@
 //
 // Work distribution: a wave takes blocks of `chunk_px` consecutive jobs of
 // one scene from the launch's global counter and hands them to its lanes as
-// they fall idle.  A scene's jobs are numbered pixel-major, the launch's
-// frames of one pixel next to each other (job j = pixel j / frames, frame
-// frame0 + j % frames), so a wave traces a few neighbouring pixels in many
-// frames at once: their primary rays are the same ray (there is no jitter)
-// and their first hits and shadow rays mostly shared, which keeps the lanes
-// of a wave on the same tree nodes and makes whole waves miss the same
-// spheres in the flat loop.  Samples are stored frame-major ([frame][pixel]), so that
-// k_accumulate reads them coalesced; a wave writes all frames of its few
-// pixels within a short time, and L2 merges those stores into whole lines.  Spheres and the scene table
+// they fall idle.  A scene's jobs come in groups of 8 neighbouring pixels
+// times the launch's frames, frame after frame inside a group
+// (job_pixel_frame), so a wave traces 8 neighbouring pixels in many frames
+// at once: their rays start from the same few hits, which keeps the lanes of
+// a wave on nearby tree nodes and makes whole waves miss the same spheres in
+// the flat loop.  Samples are stored frame-major ([frame][pixel]), so that
+// k_accumulate reads them coalesced, and the 8 lanes that run one frame of a
+// group write one whole 128-byte line together.  Bounce 0 of a path comes
+// from the per-pixel first-bounce cache (k_first_hit) when the launch has
+// one: the lane enters the trace loop at its first continuation ray.  Spheres and the scene table
 // are read with wave-uniform addresses through the constant address space,
 // which the compiler lowers to scalar loads (two spheres per s_load_dwordx8,
 // used directly as VALU operands).  A wave works on one scene at a time and
@@ -246,10 +247,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
                 const uint32_t nidle = (uint32_t)__popcll(idle);
                 if (state == kIdle && rank < avail) {
-                    // job j: pixel j / frames, frame frame0 + j % frames
                     const uint32_t j = blk0 + blk_next + rank;
-                    px = j / S->frames;
-                    frame = S->frame0 + (j - px * S->frames);
+                    job_pixel_frame(j, S->width * S->height, S->frames, px, frame);
+                    frame += S->frame0;
                     ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
                     if (S->dirs) {
                         const float4 dv = S->dirs[px];
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         const uint32_t lsel = nl > 0u ? pcg_hash(seed) % nl : 0u;
                         const uint4* fr = S->first + 4 * (size_t)px;
                         const uint4 r0 = fr[0], r1 = fr[1], r2 = fr[2], r3 = fr[3];
-                        const float4 lt = S->first_light ? S->first_light[(size_t)px * nl + lsel]
+                        const float4 lt = S->first_light ? S->first_light[(size_t)lsel * (S->width * S->height) + px]
                                                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         const int32_t hid = (int32_t)r0.x;
                         ++reused;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(256) void k_camera_dirs(CamDirArgs c)
 struct FirstArgs {
     WfScene s;
     uint4* rec;      // 4 per pixel
-    float4* light;   // n_light per pixel, or nullptr
+    float4* light;   // [light][pixel], or nullptr
 };
 
 // One lane per pixel: the primary ray's closest hit, the shading at it, and
@@ -512,8 +512,9 @@ struct FirstArgs {
 __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
 {
     const WfScene& s = a.s;
+    const uint32_t npx = s.width * s.height;
     const uint32_t px = blockIdx.x * 256u + threadIdx.x;
-    if (px >= s.width * s.height)
+    if (px >= npx)
         return;
     const f3 ro = mk(s.cam_pos[0], s.cam_pos[1], s.cam_pos[2]);
     const float4 dv = s.dirs[px];
@@ -548,12 +549,12 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
                     mask |= 1u << l;
                     term = light_term(s.mat, s.light, mid, l, nrm, rd, L, one);
                 }
-                a.light[(size_t)px * s.n_light + l] = make_float4(term.x, term.y, term.z, 0.0f);
+                a.light[(size_t)l * npx + px] = make_float4(term.x, term.y, term.z, 0.0f);
             }
         }
     } else if (a.light) {
         for (uint32_t l = 0; l < s.n_light; ++l)
-            a.light[(size_t)px * s.n_light + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.light[(size_t)l * npx + px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     uint4* r = a.rec + 4 * (size_t)px;
     r[0] = make_uint4((uint32_t)h.id, __float_as_uint(h.t), mask, (uint32_t)mid);

@@ -4,8 +4,8 @@
 //
 // One launch renders frames [frame0, frame0 + frames) of every scene in its
 // table in two kernels.  The path kernel treats every (frame, pixel) path as
-// an independent job; a scene's jobs are numbered pixel-major (job j: pixel
-// j / frames, frame frame0 + j % frames).  Waves take blocks of `chunk_px`
+// an independent job; a scene's jobs come in groups of kJobGroupPx
+// neighbouring pixels times the launch's frames (job_pixel_frame).  Waves take blocks of `chunk_px`
 // consecutive jobs of one scene from a global counter (scene s owns chunk
 // ids [chunk_begin, chunk_begin + n_chunks), n_chunks = ceil(pixels * frames
 // / chunk_px)) and hand the block's paths to their lanes one by one; a lane
@@ -58,7 +58,7 @@ struct WfScene {
     float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
-    const float4* first_light; // first-bounce light terms, n_light per pixel (nullptr: none)
+    const float4* first_light; // first-bounce light terms, [light][pixel] (nullptr: none)
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -73,6 +73,25 @@ struct WfPass {
 };
 
 constexpr uint32_t kMaxChunkPx = 1024;
+
+// Job order.  A scene's jobs come in groups of kJobGroupPx neighbouring
+// pixels (the last group may hold fewer) times the launch's frames, frame
+// after frame inside a group: jobs 8g .. 8g+7 are pixels 8g .. 8g+7 of the
+// first frame, the next 8 the same pixels in the second frame, and so on.
+// The 64 lanes of a wave then run 8 frames of 8 neighbouring pixels at a
+// time, and the 8 samples of one 128-byte line of the [frame][pixel] sample
+// buffer are written together.
+constexpr uint32_t kJobGroupPx = 8;
+__host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pixels, uint32_t frames, uint32_t& px,
+                                                         uint32_t& frame)
+{
+    const uint32_t g = j / (kJobGroupPx * frames);
+    const uint32_t w = j - g * kJobGroupPx * frames;
+    const uint32_t left = pixels - kJobGroupPx * g;
+    const uint32_t m = left < kJobGroupPx ? left : kJobGroupPx;
+    frame = w / m;
+    px = kJobGroupPx * g + (w - frame * m);
+}
 
 // Resident waves per SIMD the path kernel's registers are budgeted for:
 // launches with trees, and launches whose scenes all use the flat loop.
@@ -104,9 +123,9 @@ hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream
 //      [1] normal x, y, z, origin of the rays leaving the hit x;
 //      [2] origin y, z, radiance after the hit's emission r, g;
 //      [3] radiance b, 0, 0, 0;
-//  * then, for scenes of 1..kFirstMaxLights lights, per pixel and light the
-//    light's Cook-Torrance term at the hit (float4, w = 0; zero when the
-//    light does not reach the hit).
+//  * then, for scenes of 1..kFirstMaxLights lights, per light and pixel
+//    ([light][pixel]) the light's Cook-Torrance term at the hit (float4,
+//    w = 0; zero when the light does not reach the hit).
 // The path kernel then does bounce 0 from it where it hands out a job.
 constexpr uint32_t kFirstMaxLights = 32;
 inline size_t first_cache_bytes(size_t pixels, uint32_t lights)

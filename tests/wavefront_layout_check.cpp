@@ -1,7 +1,7 @@
 // Host-side checks of the persistent kernel's launch layout
 // (ataraxia_amd/csrc/wavefront_host.h): chunk ids, pixel and sample offsets,
-// launch totals, the 32-bit guard, the flat-loop test of a table, and the
-// chunk size chosen per launch (auto_chunk_px).
+// launch totals, the 32-bit guard, the flat-loop test of a table, the job
+// order (job_pixel_frame) and the chunk size chosen per launch (auto_chunk_px).
 // Built host-only (hipcc --offload-host-only) by tests/test_wavefront_host.py.
 #include <cstdio>
 #include <vector>
@@ -65,6 +65,33 @@ int main()
     std::vector<WfScene> one = {scene(1, 1, 1, 0)};
     CHECK(layout_launch(one.data(), one.size(), 64, base, &p));
     CHECK(one[0].n_chunks == 1 && p.total_chunks == 1 && p.total_px == 1);
+
+    // job order: every (pixel, frame) pair exactly once, for group-size
+    // multiples and remainders; 8 pixels of one frame side by side
+    const uint32_t sizes[][2] = {{1, 1}, {5, 3}, {8, 64}, {13, 7}, {1000, 1}, {1923, 64}, {4097, 5}};
+    for (const auto& sz : sizes) {
+        const uint32_t npx = sz[0], frames = sz[1];
+        std::vector<uint8_t> seen((size_t)npx * frames, 0);
+        bool ok = true;
+        for (uint32_t j = 0; j < npx * frames; ++j) {
+            uint32_t px = 0, f = 0;
+            job_pixel_frame(j, npx, frames, px, f);
+            if (px >= npx || f >= frames || seen[(size_t)f * npx + px]++) { ok = false; break; }
+        }
+        CHECK(ok);
+    }
+    for (uint32_t j = 0; j < 64; ++j) {
+        uint32_t px = 0, f = 0;
+        job_pixel_frame(8 * 64 * 3 + j, 1920 * 1080, 64, px, f);
+        CHECK(px == 24 + j % 8 && f == j / 8);
+    }
+    {
+        uint32_t px = 0, f = 0;
+        job_pixel_frame(13 * 7 - 1, 13, 7, px, f);   // last group: pixels 8..12, 5 wide
+        CHECK(px == 12 && f == 6);
+        job_pixel_frame(8 * 7 + 5, 13, 7, px, f);
+        CHECK(px == 8 && f == 1);
+    }
 
     // chunk size chosen per launch: 1080p frames on a 256-CU grid
     // (tree launches 6 workgroups per CU, flat-loop launches 7)
