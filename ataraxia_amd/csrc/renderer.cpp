@@ -38,7 +38,7 @@ struct Renderer {
     bool dirs_valid = false, first_valid = false, prev_valid = false;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
-    uint32_t wf_frames = kDefaultFramesPerLaunch;
+    uint32_t wf_frames = kRendererFramesPerLaunch;
     DevBuf accum, comp, image, counters;
     uint32_t width = 0, height = 0;
     atx_settings settings{1, 0, {0, 0}, 15};  // Scene.h:49-56 defaults

@@ -120,9 +120,18 @@ inline uint32_t auto_chunk_px(uint64_t jobs, uint32_t workgroups)
     return per_wave >= 5000u ? 512u : 256u;
 }
 
+// Frames per launch of batches and serving ticks (each scene or request
+// advances this many frames per launch).
 constexpr uint32_t kDefaultFramesPerLaunch = 64;
+// Frames per launch of the single-scene renderer.  Every launch of the
+// persistent kernel costs about 0.7 ms beyond its paths (ramp-up and
+// drain; 1080p: 10.75 ms for 64 frames, 5.73 ms for 32), so a render call
+// takes as few launches as its sample buffer allows: 1080p, 256 frames per
+// launch, 0.172 -> 0.160 ms per frame against 64.
+constexpr uint32_t kRendererFramesPerLaunch = 256;
 // Upper bound of one launch's sample buffer (16 B per pixel and frame):
 // renderers shorten their launches, batches split them by scene groups.
-constexpr size_t kSampleBufferBytes = size_t(2) << 30;
+// 8 GiB is 3 % of the device's HBM and holds 256 frames of 1080p.
+constexpr size_t kSampleBufferBytes = size_t(8) << 30;
 
 }  // namespace atx

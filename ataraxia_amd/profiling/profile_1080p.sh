@@ -3,10 +3,11 @@
 # Run from the repo root on a machine with the GPU:
 #     KERNEL=plain|wavefront ACCEL=bvh|none [OUT=dir] bash ataraxia_amd/profiling/profile_1080p.sh
 # then summarise with: python ataraxia_amd/profiling/summarize.py <OUT>
-#   1. kernel trace + stats of bench.py (STEPS frames of the 1080p workload)
+#   1. kernel trace + stats of bench.py (STEPS frames of the 1080p workload;
+#      wavefront: 256 frames, the renderer's launch length, so one launch)
 #   2. FETCH_SIZE, then WRITE_SIZE, each in its own pass, over one launch:
-#      PMC_STEPS frames (plain: one launch per frame; wavefront: 64 frames
-#      per launch, so PMC_STEPS=64 is exactly one launch)
+#      PMC_STEPS frames (plain: one launch per frame; wavefront: 256 frames
+#      per launch, so PMC_STEPS=256 is exactly one launch)
 #   3. the same two counters on copy_probe (256 MiB copy) for calibration
 #   4. SQ counters (waves, VALU instructions and lane utilisation, waits, loads)
 # Every pass has its own time limit; the script stops at the first failure.
@@ -16,8 +17,7 @@ KERNEL=${KERNEL:-wavefront}
 ACCEL=${ACCEL:-bvh}
 OUT=${OUT:-$REPO/profile_out/prof_${KERNEL}_$ACCEL}
 case "$OUT" in /*) ;; *) OUT="$REPO/$OUT" ;; esac   # the passes run from /tmp
-STEPS=${STEPS:-128}
-if [ "$KERNEL" = plain ]; then PMC_STEPS=${PMC_STEPS:-4}; else PMC_STEPS=${PMC_STEPS:-64}; fi
+if [ "$KERNEL" = plain ]; then STEPS=${STEPS:-128}; PMC_STEPS=${PMC_STEPS:-4}; else STEPS=${STEPS:-256}; PMC_STEPS=${PMC_STEPS:-256}; fi
 BENCH=("$REPO/bench.py" --kernel "$KERNEL" --accel "$ACCEL" --no-serve --no-cpu-baseline)
 mkdir -p "$OUT"
 export TMPDIR=/tmp
