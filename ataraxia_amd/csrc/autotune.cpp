@@ -1,7 +1,7 @@
 // autotune.cpp -- picks the batch renderer's chunk size and persistent grid by
 // timing candidates on the batch itself, and caches the choice.
 //
-// Candidates: chunk_px in {128, 256, 512, 1024} x frames per launch in {16, 32, 64}
+// Candidates: chunk_px in {256, 512} x frames per launch in {32, 64, 128, 256}
 // on the full resident-workgroup grid, then the winner on 3/4 of the grid.
 // Each candidate renders the first F frames of a spread-out subsample of the
 // batch's scenes (up to 16 Mpixel, at least 8 scenes) twice (the second run is
@@ -113,8 +113,8 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
         }
         return ATX_OK;
     };
-    const uint32_t chunks[] = {128u, 256u, 512u, 1024u};
-    const uint32_t frames[] = {16u, 32u, 64u};
+    const uint32_t chunks[] = {256u, 512u};
+    const uint32_t frames[] = {32u, 64u, 128u, 256u};
     for (uint32_t c : chunks) {
         for (uint32_t F : frames) {
             rc = try_one(atx_batch_tuning{c, wavefront_full_grid(batch_flat_only(b)), 1, F});

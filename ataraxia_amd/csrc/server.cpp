@@ -467,7 +467,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     if (!s) { set_error("out of host memory"); return ATX_ERR_NOMEM; }
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.frames_per_tick) s->cfg.frames_per_tick = kDefaultFramesPerLaunch;
-    if (!s->cfg.tick_mpaths) s->cfg.tick_mpaths = 32;
+    if (!s->cfg.tick_mpaths) s->cfg.tick_mpaths = 128;
     ATX_CHECK_ARG(s->cfg.chunk_px % 64 == 0 && s->cfg.chunk_px <= kMaxChunkPx, "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(s->cfg.frames_per_tick <= 4096, "frames_per_tick must be <= 4096");
     s->cfg.kahan = s->cfg.kahan ? 1u : 0u;

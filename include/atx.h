@@ -233,7 +233,7 @@ typedef struct atx_server_config {
     uint32_t kahan;             /* 1 = compensated accumulation                             */
     uint32_t accel;             /* ATX_ACCEL_BVH (0) or ATX_ACCEL_NONE                      */
     uint32_t tick_mpaths;       /* admit until a launch holds this many million paths
-                                   (0 = 32; UINT32_MAX = admit whatever fits the budget)    */
+                                   (0 = 128; UINT32_MAX = admit whatever fits the budget)   */
 } atx_server_config;
 
 typedef struct atx_server_stats {
