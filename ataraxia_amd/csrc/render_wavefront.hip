@@ -218,6 +218,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     CScene* S = nullptr;
     int32_t pend = -1;              // a block of another scene, waiting for the lanes to drain
     bool qdone = false;
+    bool first_fetch = true;
+    const uint32_t n_waves = gridDim.x * 4u;
+    const uint32_t wave_id = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
     uint32_t blk0 = 0, blk_next = 0, blk_end = 0;   // jobs left: [blk0 + next, blk0 + end) of scene S
 
     // ---- per-lane path state ----
@@ -332,10 +335,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
             if (c < 0) {
                 if (qdone)
                     break;
+                // a wave's first chunk is its own index in the grid; the
+                // counter hands out the chunks after the grid's first
+                // n_waves (all waves asking the one counter at once made the
+                // first fetch wait 35 us on average)
                 uint32_t got = 0;
-                if (lane == 0)
-                    got = atomicAdd(a.queue, 1u);
-                got = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(got, 0, 64));
+                if (first_fetch) {
+                    got = wave_id;
+                    first_fetch = false;
+                } else {
+                    if (lane == 0)
+                        got = atomicAdd(a.queue, 1u) + n_waves;
+                    got = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(got, 0, 64));
+                }
                 if (got >= a.total_chunks) {
                     qdone = true;
                     break;
