@@ -88,7 +88,7 @@ class ServerStats(C.Structure):
                 ("samples", C.c_uint64), ("ticks", C.c_uint64), ("hbm_in_use", C.c_uint64),
                 ("hbm_high_water", C.c_uint64), ("device_used_high_water", C.c_uint64),
                 ("gpu_busy_ms", C.c_double), ("admit_ms", C.c_double), ("retire_ms", C.c_double),
-                ("wait_ms", C.c_double)]
+                ("wait_ms", C.c_double), ("release_ms", C.c_double), ("gap_ms", C.c_double)]
 
 
 class RequestInfo(C.Structure):

@@ -8,6 +8,8 @@
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,30 +38,95 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
         }                                 \
     } while (0)
 
+// Device blocks kept for reuse.  hipFree waits for the device and costs
+// ~70 us per call even when it is idle, so an owner that allocates and
+// frees per request (the server) returns blocks here and takes them back
+// for later requests.  A block is handed out for a request of at least half
+// its size; past `limit` pooled bytes the largest blocks are freed.  Blocks
+// are only ever given back once the device no longer uses them.
+class DevPool {
+public:
+    explicit DevPool(size_t limit = 0) : limit_(limit) {}
+    ~DevPool() { trim(0); }
+    DevPool(const DevPool&) = delete;
+    DevPool& operator=(const DevPool&) = delete;
+    void set_limit(size_t limit) { std::lock_guard<std::mutex> lk(mu_); limit_ = limit; }
+    // A pooled block of capacity in [bytes, 2 * bytes + 64 KiB], or nullptr.
+    void* take(size_t bytes, size_t* cap)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(bytes);
+        if (it == free_.end() || it->first > 2 * bytes + (64u << 10)) return nullptr;
+        void* p = it->second;
+        *cap = it->first;
+        pooled_ -= it->first;
+        free_.erase(it);
+        return p;
+    }
+    void give(void* p, size_t cap)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(cap, p);
+        pooled_ += cap;
+        while (pooled_ > limit_ && !free_.empty()) {
+            auto big = std::prev(free_.end());
+            pooled_ -= big->first;
+            (void)hipFree(big->second);
+            free_.erase(big);
+        }
+    }
+    void trim(size_t keep)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        while (pooled_ > keep && !free_.empty()) {
+            auto big = std::prev(free_.end());
+            pooled_ -= big->first;
+            (void)hipFree(big->second);
+            free_.erase(big);
+        }
+    }
+    size_t pooled() const { std::lock_guard<std::mutex> lk(mu_); return pooled_; }
+
+private:
+    mutable std::mutex mu_;
+    std::multimap<size_t, void*> free_;   // capacity -> block
+    size_t pooled_ = 0, limit_ = 0;
+};
+
 // Move-only owner of one device allocation (the reference's CudaBuffer has a
 // defaulted copy constructor, DeviceMemory.h:33, which can double-free).
+// With a pool (use_pool), blocks come from and go back to it.
 class DevBuf {
 public:
     DevBuf() = default;
     ~DevBuf() { release(); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), pool_(o.pool_) { o.p_ = nullptr; o.n_ = 0; }
     DevBuf& operator=(DevBuf&& o) noexcept
     {
         if (this != &o) {
             release();
-            p_ = o.p_; n_ = o.n_;
+            p_ = o.p_; n_ = o.n_; pool_ = o.pool_;
             o.p_ = nullptr; o.n_ = 0;
         }
         return *this;
     }
+    void use_pool(DevPool* pool) { pool_ = pool; }
     // Grows to at least `bytes` (contents are not preserved on growth).
     hipError_t reserve(size_t bytes)
     {
         if (bytes <= n_ && p_) return hipSuccess;
         release();
         if (bytes == 0) return hipSuccess;
+        if (pool_) {
+            size_t cap = 0;
+            if (void* q = pool_->take(bytes, &cap)) {
+                p_ = q;
+                n_ = cap;
+                return hipSuccess;
+            }
+        }
         hipError_t e = hipMalloc(&p_, bytes);
         if (e != hipSuccess) { p_ = nullptr; return e; }
         n_ = bytes;
@@ -67,7 +134,10 @@ public:
     }
     void release()
     {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            if (pool_) pool_->give(p_, n_);
+            else (void)hipFree(p_);
+        }
         p_ = nullptr;
         n_ = 0;
     }
@@ -78,6 +148,7 @@ public:
 private:
     void* p_ = nullptr;
     size_t n_ = 0;
+    DevPool* pool_ = nullptr;
 };
 
 // Device-layout copy of one flattened scene (see device_types.h).
@@ -101,6 +172,11 @@ struct DevScene {
     DevBuf bvh_nodes, bvh_lsph, bvh_lidx, bvh_big;
     uint32_t n_sph = 0, n_mat = 0, n_light = 0;
     BvhDev bvh{};
+    void use_pool(DevPool* pool)
+    {
+        for (DevBuf* b : {&sph, &sph_mat, &mat, &light, &bvh_nodes, &bvh_lsph, &bvh_lidx, &bvh_big})
+            b->use_pool(pool);
+    }
     size_t bytes() const
     {
         return sph.bytes() + sph_mat.bytes() + mat.bytes() + light.bytes() + bvh_nodes.bytes() +

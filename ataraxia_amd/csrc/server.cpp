@@ -63,6 +63,9 @@ struct Request {
 };
 
 struct Server {
+    // Device blocks of released requests, reused by later ones (first member:
+    // destroyed after every request that may still hold a block).
+    DevPool pool;
     atx_server_config cfg{};
     std::chrono::steady_clock::time_point epoch;
     std::mutex mu;
@@ -83,6 +86,8 @@ struct Server {
     // worker-only device resources
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_prev = nullptr;   // end of the previous launch (gap_ms)
+    bool have_prev = false;         // the previous launch ran without an idle period since
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
     double rendered = 0.0, busy_s = 0.0;   // samples rendered and GPU seconds so far (worker)
@@ -124,6 +129,9 @@ static uint64_t tick_paths_of(const Request& r, uint32_t frames_per_tick)
 // Device allocations of one request (worker thread).
 static bool admit(Server* s, Request& r)
 {
+    r.geo.use_pool(&s->pool);
+    for (DevBuf* b : {&r.accum, &r.comp, &r.image, &r.dirs, &r.first})
+        b->use_pool(&s->pool);
     int rc = upload_scene(r.spheres.data(), r.spheres.size(), r.materials.data(), r.materials.size(),
                           r.lights.data(), r.lights.size(), &r.geo);
     const size_t npx = (size_t)r.cam.width * r.cam.height;
@@ -181,7 +189,13 @@ static void release_returned(Server* s)
         std::lock_guard<std::mutex> lk(s->mu);
         rel.swap(s->release_q);
     }
+    if (rel.empty())
+        return;
+    const auto t0 = std::chrono::steady_clock::now();
     for (auto& r : rel) release_device(s, *r);
+    const double ms = ms_since(t0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.release_ms += ms;
 }
 
 // Copies a finished request's results to host memory (ok) and marks it done
@@ -325,8 +339,11 @@ static void end_tick(Server* s, bool ok, const std::vector<WfScene>& table,
     const auto t_wait0 = std::chrono::steady_clock::now();
     ok = hipStreamSynchronize(s->stream) == hipSuccess && ok;
     const double wms = ms_since(t_wait0);
-    float ms = 0.0f;
+    float ms = 0.0f, gap = 0.0f;
     if (ok) (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    if (ok && s->have_prev) (void)hipEventElapsedTime(&gap, s->ev_prev, s->ev0);
+    std::swap(s->ev1, s->ev_prev);
+    s->have_prev = ok;
     if (ok) {
         s->busy_s += ms * 1e-3;
         for (const WfScene& w : table) s->rendered += (double)w.width * w.height * w.frames;
@@ -339,6 +356,7 @@ static void end_tick(Server* s, bool ok, const std::vector<WfScene>& table,
         s->stats.ticks++;
         s->stats.gpu_busy_ms += ms;
         s->stats.wait_ms += wms;
+        s->stats.gap_ms += gap;
         if (total_b) s->stats.device_used_high_water = std::max<uint64_t>(s->stats.device_used_high_water, total_b - free_b);
         for (auto& r : in_tick) {
             if (!ok) {
@@ -422,6 +440,7 @@ static void worker_main(Server* s)
         if (idle) {
             release_returned(s);   // nothing runs: free retired requests' memory, then admit
             admit_queued(s);
+            s->have_prev = false;  // the device idled for want of work: no gap to count
         }
         bool launched = false, ok = false;
         {
@@ -479,9 +498,12 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
         ATX_HIP(hipMemGetInfo(&free_b, &total_b));
         s->cfg.hbm_budget_bytes = free_b / 2;
     }
+    // released requests' blocks kept for reuse: at most a quarter of the budget
+    s->pool.set_limit(s->cfg.hbm_budget_bytes / 4);
     ATX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     ATX_HIP(hipEventCreate(&s->ev0));
     ATX_HIP(hipEventCreate(&s->ev1));
+    ATX_HIP(hipEventCreate(&s->ev_prev));
     ATX_HIP(s->queue_ctr.reserve(256));
     ATX_HIP(s->counters.reserve(kCtrCount * sizeof(unsigned long long)));
     ATX_HIP(hipMemset(s->counters.get(), 0, kCtrCount * sizeof(unsigned long long)));
@@ -535,6 +557,7 @@ int atx_server_destroy(atx_server* h)
     s->all.clear();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->ev_prev) (void)hipEventDestroy(s->ev_prev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return ATX_OK;

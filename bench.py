@@ -301,7 +301,8 @@ def bench_serve(args, rate=None, requests=None):
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "hbm_high_water_bytes": st["hbm_high_water"],
             "device_used_high_water_bytes": st["device_used_high_water"], "ticks": st["ticks"],
             "gpu_busy_ms": st["gpu_busy_ms"], "worker_admit_ms": st["admit_ms"], "worker_retire_ms": st["retire_ms"],
-            "worker_wait_ms": st["wait_ms"], "frames_per_tick": args.frames_per_tick or "default",
+            "worker_wait_ms": st["wait_ms"], "worker_release_ms": st["release_ms"], "gap_ms": st["gap_ms"],
+            "frames_per_tick": args.frames_per_tick or "default",
             "tick_mpaths": args.tick_mpaths or "default",
             "requests_spec": "resolution from {512x512, 1280x720, 1920x1080}, spheres logU[4,1024] + ground, "
                              "spp logU[16,1024], 8 bounces, 1..8 lights"}

@@ -249,6 +249,10 @@ typedef struct atx_server_stats {
     double retire_ms;              /* time retiring them (result copies), on a
                                       thread of its own                         */
     double wait_ms;                /* worker time waiting for launches to end   */
+    double release_ms;             /* worker time releasing finished requests'
+                                      device buffers (between launches)         */
+    double gap_ms;                 /* device time between the end of a launch
+                                      and the start of the next one, summed     */
 } atx_server_stats;
 
 typedef struct atx_request_info {
