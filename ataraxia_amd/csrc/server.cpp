@@ -87,6 +87,11 @@ struct Server {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_prev = nullptr;   // end of the previous launch (gap_ms)
+    // admission kernels (primary rays, first-bounce caches) run on their own
+    // stream, beside the running launch; the next launch waits for adm_ev
+    hipStream_t adm_stream = nullptr;
+    hipEvent_t adm_ev = nullptr;
+    bool adm_pending = false;
     bool have_prev = false;         // the previous launch ran without an idle period since
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
@@ -141,20 +146,24 @@ static bool admit(Server* s, Request& r)
                          (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
         rc = ATX_ERR_NOMEM;
     if (rc == ATX_OK) {
-        // on the worker's stream, ahead of the request's first launch
+        // on the admission stream, beside the running launch; the request's
+        // first launch waits for it (adm_ev)
         WfScene w;
         fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, (int)s->cfg.accel, &w);
         w.dirs = r.dirs.as<float4>();
         if (r.spp >= 2) set_first_cache(w, r.first.get());
         // the first-bounce cache pays from two frames per request on
-        if (launch_camera_dirs(w, r.dirs.as<float4>(), s->stream) != hipSuccess ||
-            (r.spp >= 2 && launch_first_hits(w, s->stream) != hipSuccess)) {
+        if (launch_camera_dirs(w, r.dirs.as<float4>(), s->adm_stream) != hipSuccess ||
+            (r.spp >= 2 && launch_first_hits(w, s->adm_stream) != hipSuccess)) {
             set_error("primary rays: launch failed");
             rc = ATX_ERR_HIP;
         }
     }
     if (rc != ATX_OK) {
         r.error = atx_last_error();
+        // a kernel of this request may be queued on the admission stream:
+        // let it finish before its blocks go back to the pool
+        (void)hipStreamSynchronize(s->adm_stream);
         r.geo = DevScene();
         r.accum.release();
         r.comp.release();
@@ -282,6 +291,10 @@ static void admit_queued(Server* s)
             finish(s, r, false, true);
         }
     }
+    if (hipEventRecord(s->adm_ev, s->adm_stream) == hipSuccess)
+        s->adm_pending = true;
+    else
+        (void)hipStreamSynchronize(s->adm_stream);
     const double ms = ms_since(t0);
     std::lock_guard<std::mutex> lk(s->mu);
     s->stats.admit_ms += ms;
@@ -314,6 +327,10 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
     const uint32_t P =
         s->cfg.chunk_px ? s->cfg.chunk_px : auto_chunk_px(launch_samples(table.data(), table.size()), shape.workgroups);
     if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
+    if (ok && s->adm_pending) {
+        ok = hipStreamWaitEvent(s->stream, s->adm_ev, 0) == hipSuccess;
+        s->adm_pending = false;
+    }
     ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
          hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice,
                         s->stream) == hipSuccess &&
@@ -501,6 +518,8 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     // released requests' blocks kept for reuse: at most a quarter of the budget
     s->pool.set_limit(s->cfg.hbm_budget_bytes / 4);
     ATX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    ATX_HIP(hipStreamCreateWithFlags(&s->adm_stream, hipStreamNonBlocking));
+    ATX_HIP(hipEventCreateWithFlags(&s->adm_ev, hipEventDisableTiming));
     ATX_HIP(hipEventCreate(&s->ev0));
     ATX_HIP(hipEventCreate(&s->ev1));
     ATX_HIP(hipEventCreate(&s->ev_prev));
@@ -552,13 +571,16 @@ int atx_server_destroy(atx_server* h)
     s->cv_retire.notify_all();
     if (s->retirer.joinable()) s->retirer.join();
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->adm_stream) (void)hipStreamSynchronize(s->adm_stream);
     s->active.clear();
     s->queue.clear();
     s->all.clear();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->ev_prev) (void)hipEventDestroy(s->ev_prev);
+    if (s->adm_ev) (void)hipEventDestroy(s->adm_ev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->adm_stream) (void)hipStreamDestroy(s->adm_stream);
     delete s;
     return ATX_OK;
 }
