@@ -28,6 +28,7 @@ struct BatchEntry {
     DevBuf accum, comp, image;
     DevBuf dirs;                  // primary ray direction per pixel (written once)
     DevBuf first;                 // first-bounce cache per pixel (written once)
+    uint32_t n_hit = 0;           // pixels of its hit list
     bool dirs_ready = false;
     atx_settings settings{};
     atx_camera cam{};
@@ -51,6 +52,7 @@ struct Batch {
     std::vector<uint32_t> launch_chunks;  // chunk ids of launch l
     std::vector<uint32_t> launch_chunk_px;  // jobs per chunk of launch l
     uint32_t launches = 0, chunk_px = 0, frames_per_launch = 0, grid = 0, max_spp = 0;
+    uint64_t skipped = 0;   // primary rays of pixels left out of the jobs (hit lists), per render
     bool flat_only = false;   // no scene has a tree: the flat-loop kernel instance
     bool kahan = false;
     DevBuf d_samples;                     // sample buffer, shared by the launches (stream order)
@@ -149,7 +151,10 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             if (cache) set_first_cache(w, e.first.get());
             if (!e.dirs_ready) {
                 ATX_HIP(launch_camera_dirs(w, e.dirs.as<float4>(), b->stream));
-                if (cache) ATX_HIP(launch_first_hits(w, b->stream));
+                if (cache) {
+                    ATX_HIP(launch_first_hits(w, b->stream));
+                    ATX_HIP(read_first_hits(e.first.get(), npx, w.n_light, b->stream, &e.n_hit));
+                }
                 e.dirs_ready = true;
             }
             if (e.capture) {
@@ -158,6 +163,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
                 w.dbg_shadow = e.dbg_sh.as<int32_t>();
                 w.dbg_bounces = e.dbg_bounces;
             }
+            if (cache) use_hit_list(w, e.first.get(), e.n_hit);
             table.push_back(w);
         }
         b->launch_scenes.push_back((uint32_t)(table.size() - b->launch_off.back()));
@@ -169,7 +175,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
         const WfScene* lt = table.data() + b->launch_off[l];
-        const uint32_t P = b->chunk_px ? b->chunk_px : auto_chunk_px(launch_samples(lt, b->launch_scenes[l]), b->grid);
+        const uint32_t P = b->chunk_px ? b->chunk_px : auto_chunk_px(launch_jobs(lt, b->launch_scenes[l]), b->grid);
         WfPass lp{};
         if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], P,
                            b->d_samples.as<float4>(), &lp)) {
@@ -181,6 +187,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
         b->launch_px.push_back(lp.total_px);
     }
     b->launches = (uint32_t)b->launch_off.size();
+    b->skipped = skipped_primaries(table.data(), table.size());
     ATX_HIP(b->d_table.reserve(std::max<size_t>(1, table.size()) * sizeof(WfScene)));
     if (!table.empty())
         ATX_HIP(hipMemcpy(b->d_table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice));
@@ -234,8 +241,8 @@ int batch_run(Batch* b, uint32_t max_frames, double* ms_out)
         samples += (uint64_t)e.cam.width * e.cam.height * frames;
     }
     b->stats.samples = samples;
-    b->stats.rays = ctr[kCtrRays] + ctr[kCtrReused];
-    b->stats.reused_rays = ctr[kCtrReused];
+    b->stats.rays = ctr[kCtrRays] + ctr[kCtrReused] + b->skipped;
+    b->stats.reused_rays = ctr[kCtrReused] + b->skipped;
     b->stats.sphere_tests = ctr[kCtrTests];
     b->stats.exact_retraces = ctr[kCtrExact];
     b->stats.box_tests = ctr[kCtrBoxes];

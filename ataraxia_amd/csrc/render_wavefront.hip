@@ -251,7 +251,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const uint32_t nidle = (uint32_t)__popcll(idle);
                 if (state == kIdle && rank < avail) {
                     const uint32_t j = blk0 + blk_next + rank;
-                    job_pixel_frame(j, S->width * S->height, S->frames, px, frame);
+                    job_pixel_frame(j, S->hit_px ? S->n_hit : S->width * S->height, S->frames, px, frame);
+                    if (S->hit_px)
+                        px = S->hit_px[px];
                     frame += S->frame0;
                     ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
                     if (S->dirs) {
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 S = &scenes[s];
             }
             pend = -1;
-            const uint32_t jobs = S->width * S->height * S->frames;
+            const uint32_t jobs = (S->hit_px ? S->n_hit : S->width * S->height) * S->frames;
             blk0 = ((uint32_t)c - S->chunk_begin) * P;
             blk_end = blk0 >= jobs ? 0u : (jobs - blk0 < P ? jobs - blk0 : P);
             blk_next = 0;
@@ -466,11 +468,15 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
     if (px >= npx)
         return;
     const bool first = S.frame0 == 1u;
+    // a pixel left out of the jobs (its primary ray misses): every frame's
+    // sample is the one such a path returns, the sky colour or black
+    const bool sky = S.hit_px && (int32_t)S.first[4 * (size_t)px].x < 0;
+    const float4 sky_c = S.sky_light ? make_float4(0.6f, 0.7f, 0.9f, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     float4 acc = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.accum[px];
     if (S.kahan) {
         float4 cp = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.comp[px];
         for (uint32_t f = 0; f < S.frames; ++f) {
-            const float4 c = S.samples[(size_t)f * npx + px];
+            const float4 c = sky ? sky_c : S.samples[(size_t)f * npx + px];
             float yv, t;
             yv = c.x - cp.x; t = acc.x + yv; cp.x = (t - acc.x) - yv; acc.x = t;
             yv = c.y - cp.y; t = acc.y + yv; cp.y = (t - acc.y) - yv; acc.y = t;
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
         S.comp[px] = cp;
     } else {
         for (uint32_t f = 0; f < S.frames; ++f) {
-            const float4 c = S.samples[(size_t)f * npx + px];
+            const float4 c = sky ? sky_c : S.samples[(size_t)f * npx + px];
             acc.x = acc.x + c.x;
             acc.y = acc.y + c.y;
             acc.z = acc.z + c.z;
@@ -512,8 +518,10 @@ __global__ __launch_bounds__(256) void k_camera_dirs(CamDirArgs c)
 
 struct FirstArgs {
     WfScene s;
-    uint4* rec;      // 4 per pixel
-    float4* light;   // [light][pixel], or nullptr
+    uint4* rec;        // 4 per pixel
+    float4* light;     // [light][pixel], or nullptr
+    uint32_t* hits;    // pixels whose primary ray hits, appended
+    uint32_t* n_hits;  // their count (zero before the launch)
 };
 
 // One lane per pixel: the primary ray's closest hit, the shading at it, and
@@ -541,6 +549,21 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
         return trace_filtered(flat, s.n_sph, o, d, ex);
     };
     const Hit h = trace(ro, rd, -1.0f);
+    {
+        // append this wave's hit pixels to the list (one atomic per wave)
+        const uint32_t lane = threadIdx.x & 63u;
+        const unsigned long long act = __ballot(1);
+        const unsigned long long hm = __ballot(h.id >= 0);
+        const uint32_t leader = (uint32_t)__ffsll((long long)act) - 1u;
+        uint32_t base = 0;
+        if (lane == leader && hm)
+            base = atomicAdd(a.n_hits, (uint32_t)__popcll(hm));
+        base = (uint32_t)__shfl((int)base, (int)leader, 64);
+        if (h.id >= 0) {
+            const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+            a.hits[base + (uint32_t)__popcll(hm & below)] = px;
+        }
+    }
     const f3 one = mk(1.0f, 1.0f, 1.0f);
     const f3 zero = mk(0.0f, 0.0f, 0.0f);
     f3 nrm = zero, so = zero, col = zero;
@@ -585,13 +608,14 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 
 hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream, hipEvent_t paths_done)
 {
-    if (p.total_chunks == 0)
+    if (p.total_px == 0)
         return hipSuccess;
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
-    if (l.flat_only)
+    // no jobs (every pixel's primary ray misses): k_accumulate alone
+    if (p.total_chunks > 0 && l.flat_only)
         hipLaunchKernelGGL(k_render_wavefront<kWfWavesFlat>, dim3(l.workgroups), dim3(256), 0, stream, p);
-    else
+    else if (p.total_chunks > 0)
         hipLaunchKernelGGL(k_render_wavefront<kWfWavesTree>, dim3(l.workgroups), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && paths_done)
@@ -627,10 +651,14 @@ hipError_t launch_first_hits(const WfScene& s, hipStream_t stream)
     const bool lights = s.n_light >= 1 && s.n_light <= kFirstMaxLights;
     if (!s.dirs || !s.first || lights != (s.first_light != nullptr))
         return hipErrorInvalidValue;
+    const FirstCacheView v = first_cache_view(const_cast<uint4*>(s.first), n, s.n_light);
     FirstArgs a;
     a.s = s;
-    a.rec = const_cast<uint4*>(s.first);
-    a.light = const_cast<float4*>(s.first_light);
+    a.rec = v.rec;
+    a.light = v.light;
+    a.hits = v.hits;
+    a.n_hits = v.n_hits;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, v.n_hits, 1u);
     hipLaunchKernelGGL(k_first_hit, dim3((n + 255u) / 256u), dim3(256), 0, stream, a);
     return hipGetLastError();
 }

@@ -36,6 +36,7 @@ struct Renderer {
     // cache of that camera and the current scene (set_scene clears first_valid)
     atx_camera dirs_cam{}, prev_cam{};
     bool dirs_valid = false, first_valid = false, prev_valid = false;
+    uint32_t n_hit = 0;   // pixels of wf_first's hit list
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kRendererFramesPerLaunch;
@@ -136,15 +137,6 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
         shape.flat_only = wf_flat_only(table.data(), table.size());
         shape.workgroups = wavefront_full_grid(shape.flat_only);
-        for (WfScene& e : table) {
-            const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(launch_samples(&e, 1), shape.workgroups);
-            launch_chunk.push_back(P);
-            WfPass lp{};
-            if (!layout_launch(&e, 1, P, r->wf_samples.as<float4>(), &lp)) {
-                set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
-                return ATX_ERR_INVALID;
-            }
-        }
         // The first-bounce cache pays from two frames per camera on: a call
         // of one frame uses it once the camera has stayed the same since the
         // previous call (a camera that moves every frame never builds it).
@@ -156,6 +148,29 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         for (WfScene& e : table) {
             e.dirs = r->wf_dirs.as<float4>();
             if (use_first) set_first_cache(e, r->wf_first.get());
+        }
+        // primary rays once per camera, the first-bounce cache (and its hit
+        // count, which the layout needs) once per camera and scene
+        if (!r->dirs_valid || memcmp(&r->dirs_cam, cam, sizeof(*cam)) != 0) {
+            r->dirs_valid = r->first_valid = false;
+            ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
+            r->dirs_cam = *cam;
+            r->dirs_valid = true;
+        }
+        if (use_first && !r->first_valid) {
+            ATX_HIP(launch_first_hits(table[0], r->stream));
+            ATX_HIP(read_first_hits(r->wf_first.get(), npx, r->geo.n_light, r->stream, &r->n_hit));
+            r->first_valid = true;
+        }
+        for (WfScene& e : table) {
+            if (use_first) use_hit_list(e, r->wf_first.get(), r->n_hit);
+            const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(scene_jobs(e), shape.workgroups);
+            launch_chunk.push_back(P);
+            WfPass lp{};
+            if (!layout_launch(&e, 1, P, r->wf_samples.as<float4>(), &lp)) {
+                set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
+                return ATX_ERR_INVALID;
+            }
         }
         ATX_HIP(r->wf_table.reserve(table.size() * sizeof(WfScene)));
         ATX_HIP(r->wf_queue.reserve(table.size() * sizeof(unsigned int)));
@@ -172,17 +187,6 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     if (wf) ATX_HIP(launch_zero_u32(r->wf_queue.as<unsigned int>(), (uint32_t)table.size(), r->stream));
     ATX_HIP(hipEventRecord(r->ev0, r->stream));
     if (wf) {
-        // primary rays once per camera, the first-bounce cache once per camera and scene
-        if (!r->dirs_valid || memcmp(&r->dirs_cam, cam, sizeof(*cam)) != 0) {
-            r->dirs_valid = r->first_valid = false;
-            ATX_HIP(launch_camera_dirs(table[0], r->wf_dirs.as<float4>(), r->stream));
-            r->dirs_cam = *cam;
-            r->dirs_valid = true;
-        }
-        if (use_first && !r->first_valid) {
-            ATX_HIP(launch_first_hits(table[0], r->stream));
-            r->first_valid = true;
-        }
         for (size_t l = 0; l < table.size(); ++l) {
             WfPass wp{};
             wp.scenes = r->wf_table.as<WfScene>() + l;
@@ -228,8 +232,9 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         }
     }
     r->stats.samples = (uint64_t)npx * frames;
-    r->stats.rays = ctr[kCtrRays] + ctr[kCtrReused];
-    r->stats.reused_rays = ctr[kCtrReused];
+    const uint64_t skipped = wf ? skipped_primaries(table.data(), table.size()) : 0;
+    r->stats.rays = ctr[kCtrRays] + ctr[kCtrReused] + skipped;
+    r->stats.reused_rays = ctr[kCtrReused] + skipped;
     r->stats.sphere_tests = ctr[kCtrTests];
     r->stats.exact_retraces = ctr[kCtrExact];
     r->stats.box_tests = ctr[kCtrBoxes];

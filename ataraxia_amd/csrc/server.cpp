@@ -53,6 +53,7 @@ struct Request {
     DevBuf accum, comp, image;
     DevBuf dirs;                    // primary ray direction per pixel
     DevBuf first;                   // first-bounce cache per pixel (launch_first_hits)
+    uint32_t n_hit = 0;             // pixels of its hit list
     uint32_t frames_done = 0;
     // results
     std::vector<uint32_t> rgba;
@@ -88,10 +89,10 @@ struct Server {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_prev = nullptr;   // end of the previous launch (gap_ms)
     // admission kernels (primary rays, first-bounce caches) run on their own
-    // stream, beside the running launch; the next launch waits for adm_ev
+    // stream, beside the running launch; the worker waits for them (not for
+    // the launch) to read the caches' hit counts
     hipStream_t adm_stream = nullptr;
-    hipEvent_t adm_ev = nullptr;
-    bool adm_pending = false;
+    bool no_hit_lists = false;      // a hit count could not be read: jobs cover every pixel
     bool have_prev = false;         // the previous launch ran without an idle period since
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
@@ -146,8 +147,8 @@ static bool admit(Server* s, Request& r)
                          (s->cfg.kahan && r.comp.reserve(npx * 16) != hipSuccess)))
         rc = ATX_ERR_NOMEM;
     if (rc == ATX_OK) {
-        // on the admission stream, beside the running launch; the request's
-        // first launch waits for it (adm_ev)
+        // on the admission stream, beside the running launch (admit_queued
+        // waits for it before the request's first launch)
         WfScene w;
         fill_wf_scene(r.geo, r.settings, r.cam, nullptr, nullptr, nullptr, false, 1u, 1u, (int)s->cfg.accel, &w);
         w.dirs = r.dirs.as<float4>();
@@ -291,10 +292,20 @@ static void admit_queued(Server* s)
             finish(s, r, false, true);
         }
     }
-    if (hipEventRecord(s->adm_ev, s->adm_stream) == hipSuccess)
-        s->adm_pending = true;
-    else
-        (void)hipStreamSynchronize(s->adm_stream);
+    // the hit counts of the new first-bounce caches (the next launch's layout
+    // needs them): wait for the admission stream, not for the running launch
+    bool counts_ok = hipStreamSynchronize(s->adm_stream) == hipSuccess;
+    for (auto& r : newly) {
+        if (r->state != ATX_REQ_RUNNING || r->spp < 2) continue;
+        const size_t npx = (size_t)r->cam.width * r->cam.height;
+        const FirstCacheView v = first_cache_view(r->first.get(), npx, (uint32_t)r->lights.size());
+        counts_ok = counts_ok && hipMemcpy(&r->n_hit, v.n_hits, sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!counts_ok) {
+        // without the counts every request renders all of its pixels
+        for (auto& r : newly) r->n_hit = 0;
+        s->no_hit_lists = true;
+    }
     const double ms = ms_since(t0);
     std::lock_guard<std::mutex> lk(s->mu);
     s->stats.admit_ms += ms;
@@ -316,7 +327,10 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
                       s->cfg.kahan ? r->comp.as<float4>() : nullptr, r->image.as<uint32_t>(), s->cfg.kahan != 0,
                       r->frames_done + 1u, std::min(F, r->spp - r->frames_done), (int)s->cfg.accel, &w);
         w.dirs = r->dirs.as<float4>();
-        if (r->spp >= 2) set_first_cache(w, r->first.get());
+        if (r->spp >= 2) {
+            set_first_cache(w, r->first.get());
+            if (!s->no_hit_lists) use_hit_list(w, r->first.get(), r->n_hit);
+        }
         table.push_back(w);
     }
     WfPass wp{};
@@ -325,12 +339,8 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
     const bool flat = wf_flat_only(table.data(), table.size());
     const WfLaunch shape{flat ? s->grid_flat : s->grid, flat};
     const uint32_t P =
-        s->cfg.chunk_px ? s->cfg.chunk_px : auto_chunk_px(launch_samples(table.data(), table.size()), shape.workgroups);
+        s->cfg.chunk_px ? s->cfg.chunk_px : auto_chunk_px(launch_jobs(table.data(), table.size()), shape.workgroups);
     if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
-    if (ok && s->adm_pending) {
-        ok = hipStreamWaitEvent(s->stream, s->adm_ev, 0) == hipSuccess;
-        s->adm_pending = false;
-    }
     ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
          hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice,
                         s->stream) == hipSuccess &&
@@ -519,7 +529,6 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     s->pool.set_limit(s->cfg.hbm_budget_bytes / 4);
     ATX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     ATX_HIP(hipStreamCreateWithFlags(&s->adm_stream, hipStreamNonBlocking));
-    ATX_HIP(hipEventCreateWithFlags(&s->adm_ev, hipEventDisableTiming));
     ATX_HIP(hipEventCreate(&s->ev0));
     ATX_HIP(hipEventCreate(&s->ev1));
     ATX_HIP(hipEventCreate(&s->ev_prev));
@@ -578,7 +587,6 @@ int atx_server_destroy(atx_server* h)
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->ev_prev) (void)hipEventDestroy(s->ev_prev);
-    if (s->adm_ev) (void)hipEventDestroy(s->adm_ev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->adm_stream) (void)hipStreamDestroy(s->adm_stream);
     delete s;

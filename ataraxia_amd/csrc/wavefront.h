@@ -59,6 +59,13 @@ struct WfScene {
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
     const float4* first_light; // first-bounce light terms, [light][pixel] (nullptr: none)
+    // With a first-bounce cache, the pixels whose primary ray hits a sphere
+    // (n_hit of them): the launch's jobs cover only these, and k_accumulate
+    // adds the others' sample, the sky colour (or black) of every frame,
+    // without a path.  nullptr: every pixel is a job.  Not with the debug
+    // capture or max_bounces <= 0.
+    const uint32_t* hit_px;
+    uint32_t n_hit;
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
 };
 
@@ -125,23 +132,44 @@ hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream
 //      [3] radiance b, 0, 0, 0;
 //  * then, for scenes of 1..kFirstMaxLights lights, per light and pixel
 //    ([light][pixel]) the light's Cook-Torrance term at the hit (float4,
-//    w = 0; zero when the light does not reach the hit).
+//    w = 0; zero when the light does not reach the hit);
+//  * then the list of pixels whose primary ray hits (uint32 each, in no
+//    particular order) and its length (uint32, 16-byte slot).
 // The path kernel then does bounce 0 from it where it hands out a job.
 constexpr uint32_t kFirstMaxLights = 32;
+struct FirstCacheView {
+    uint4* rec;
+    float4* light;
+    uint32_t* hits;
+    uint32_t* n_hits;
+};
+inline size_t first_light_bytes(size_t pixels, uint32_t lights)
+{
+    return lights >= 1 && lights <= kFirstMaxLights ? pixels * lights * 16 : 0;
+}
 inline size_t first_cache_bytes(size_t pixels, uint32_t lights)
 {
-    return pixels * 64 + (lights >= 1 && lights <= kFirstMaxLights ? pixels * lights * 16 : 0);
+    return pixels * 64 + first_light_bytes(pixels, lights) + ((pixels * 4 + 15) & ~size_t(15)) + 16;
+}
+inline FirstCacheView first_cache_view(void* buf, size_t pixels, uint32_t lights)
+{
+    char* b = (char*)buf;
+    FirstCacheView v;
+    v.rec = (uint4*)b;
+    v.light = first_light_bytes(pixels, lights) ? (float4*)(b + pixels * 64) : nullptr;
+    v.hits = (uint32_t*)(b + pixels * 64 + first_light_bytes(pixels, lights));
+    v.n_hits = (uint32_t*)((char*)v.hits + ((pixels * 4 + 15) & ~size_t(15)));
+    return v;
 }
 // Points s.first / s.first_light into buf (first_cache_bytes of s's pixels and lights).
 inline void set_first_cache(WfScene& s, void* buf)
 {
-    const size_t npx = (size_t)s.width * s.height;
-    s.first = (const uint4*)buf;
-    s.first_light = (s.n_light >= 1 && s.n_light <= kFirstMaxLights) ? (const float4*)((char*)buf + npx * 64)
-                                                                      : nullptr;
+    const FirstCacheView v = first_cache_view(buf, (size_t)s.width * s.height, s.n_light);
+    s.first = v.rec;
+    s.first_light = v.light;
 }
-// Writes the cache s.first / s.first_light point to (s.dirs must hold the
-// primary directions).
+// Writes the cache s.first points to (s.dirs must hold the primary
+// directions), the hit list included.
 hipError_t launch_first_hits(const WfScene& s, hipStream_t stream);
 // Max resident 256-lane workgroups of the path kernel per CU.
 int wavefront_max_blocks_per_cu(bool flat_only);

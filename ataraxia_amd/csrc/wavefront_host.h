@@ -51,6 +51,53 @@ inline size_t launch_samples(const WfScene* tab, size_t n)
     return s;
 }
 
+// Paths (jobs) of one table entry: every pixel's, or only those of the hit
+// list (hit_px) in every frame.
+inline uint64_t scene_jobs(const WfScene& w)
+{
+    return (w.hit_px ? (uint64_t)w.n_hit : (uint64_t)w.width * w.height) * w.frames;
+}
+inline uint64_t launch_jobs(const WfScene* tab, size_t n)
+{
+    uint64_t s = 0;
+    for (size_t k = 0; k < n; ++k)
+        s += scene_jobs(tab[k]);
+    return s;
+}
+
+// Reads the hit count k_first_hit left in buf (a first-bounce cache of
+// `pixels` pixels and `lights` lights); waits for `stream`.
+inline hipError_t read_first_hits(void* buf, size_t pixels, uint32_t lights, hipStream_t stream, uint32_t* n)
+{
+    const FirstCacheView v = first_cache_view(buf, pixels, lights);
+    hipError_t e = hipMemcpyAsync(n, v.n_hits, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
+// Primary rays of the pixels a table entry leaves out of its jobs (hit
+// list): the reference traces them (they miss), so statistics count them as
+// rays answered from the cache.
+inline uint64_t skipped_primaries(const WfScene* tab, size_t n)
+{
+    uint64_t s = 0;
+    for (size_t k = 0; k < n; ++k)
+        if (tab[k].hit_px)
+            s += ((uint64_t)tab[k].width * tab[k].height - tab[k].n_hit) * tab[k].frames;
+    return s;
+}
+
+// Restricts a table entry's jobs to the cache's hit list: only with a
+// first-bounce cache, paths of at least one bounce, and no debug capture
+// (the capture records every pixel).
+inline void use_hit_list(WfScene& w, void* buf, uint32_t n_hit)
+{
+    if (!w.first || w.max_bounces <= 0 || w.dbg_radiance || w.dbg_hit || w.dbg_shadow)
+        return;
+    w.hit_px = first_cache_view(buf, (size_t)w.width * w.height, w.n_light).hits;
+    w.n_hit = n_hit;
+}
+
 // Gives the launch's table entries their chunk ids (ceil(pixels * frames /
 // chunk_px) chunks of chunk_px jobs each), pixel offsets and slices of the
 // sample buffer (in table order) and fills the launch's totals.  False when
@@ -64,7 +111,7 @@ inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* sam
     for (size_t k = 0; k < n; ++k) {
         WfScene& w = tab[k];
         const uint64_t npx = (uint64_t)w.width * w.height;
-        const uint64_t jobs = npx * w.frames;
+        const uint64_t jobs = scene_jobs(w);
         const uint64_t nch = (jobs + chunk_px - 1) / chunk_px;
         if (jobs >= kLimit || chunks + nch >= kLimit || px + npx >= kLimit)
             return false;
@@ -74,7 +121,7 @@ inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* sam
         w.samples = samples + off;
         chunks += nch;
         px += npx;
-        off += (size_t)jobs;
+        off += (size_t)(npx * w.frames);
     }
     p->total_chunks = (uint32_t)chunks;
     p->total_px = (uint32_t)px;

@@ -4,7 +4,9 @@ A pixel's primary ray has no jitter, so it is the same ray in every frame,
 and so is the shadow ray from its hit towards each light.  The persistent
 kernel takes bounce 0 of every path from a per-pixel cache (closest primary
 hit + one shadow-decision bit per light, scenes of at most 32 lights) that
-k_first_hit computes once per camera and scene (the renderer builds it for a
+k_first_hit computes once per camera and scene, with the list of pixels
+whose primary ray hits: the launch's jobs cover only those, and the others'
+sample (sky colour or black) is added by the accumulate kernel (the renderer builds it for a
 call of two or more frames, or once the camera has not moved since the
 previous call; batches and the server for scenes of two or more frames).
 Every case here renders
@@ -18,7 +20,8 @@ import pytest
 
 import oracle as O
 from ataraxia_amd import ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT, Camera, Renderer, Settings, plain_cpu_scene
-from ataraxia_amd.scene import FlatScene, SplitMix64, make_lights, random_scene, scene_from_camera_overrides
+from ataraxia_amd.scene import (FlatScene, SplitMix64, make_lights, make_materials, make_spheres, random_scene,
+                                scene_from_camera_overrides)
 from _scenes import oracle_accumulate
 
 pytestmark = pytest.mark.gpu
@@ -47,6 +50,13 @@ def _cases():
         "zero-bounces": FlatScene(base.spheres, base.materials, base.lights, Settings(True, False, 0), base.camera,
                                   name="zero-bounces"),
         "emissive": FlatScene(base.spheres, emis, base.lights, Settings(True, True, 6), base.camera, name="emissive"),
+        # no sphere: every primary ray misses, no path job at all, k_accumulate alone
+        "all-sky": FlatScene(make_spheres(0), make_materials(0), base.lights, Settings(True, True, 5), base.camera,
+                             name="all-sky"),
+        # looking up: most pixels miss; sky light off, so their samples are black
+        "mostly-miss": FlatScene(base.spheres, base.materials, base.lights, Settings(True, False, 5),
+                                 Camera.looking_at((0.0, 1.0, 6.0), target=(0.0, 9.0, -4.0), width=64, height=48),
+                                 name="mostly-miss"),
     }
 
 
