@@ -484,9 +484,30 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
             yv = c.w - cp.w; t = acc.w + yv; cp.w = (t - acc.w) - yv; acc.w = t;
         }
         S.comp[px] = cp;
-    } else {
+    } else if (sky) {
         for (uint32_t f = 0; f < S.frames; ++f) {
-            const float4 c = sky ? sky_c : S.samples[(size_t)f * npx + px];
+            acc.x = acc.x + sky_c.x;
+            acc.y = acc.y + sky_c.y;
+            acc.z = acc.z + sky_c.z;
+            acc.w = acc.w + sky_c.w;
+        }
+    } else {
+        // four frames' loads in flight ahead of their additions (in frame order)
+        const float4* sp = S.samples + px;
+        uint32_t f = 0;
+        for (; f + 4u <= S.frames; f += 4u) {
+            float4 c[4];
+            for (int k = 0; k < 4; ++k)
+                c[k] = sp[(size_t)(f + k) * npx];
+            for (int k = 0; k < 4; ++k) {
+                acc.x = acc.x + c[k].x;
+                acc.y = acc.y + c[k].y;
+                acc.z = acc.z + c[k].z;
+                acc.w = acc.w + c[k].w;
+            }
+        }
+        for (; f < S.frames; ++f) {
+            const float4 c = sp[(size_t)f * npx];
             acc.x = acc.x + c.x;
             acc.y = acc.y + c.y;
             acc.z = acc.z + c.z;
