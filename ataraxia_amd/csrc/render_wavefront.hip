@@ -256,7 +256,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         px = S->hit_px[px];
                     frame += S->frame0;
                     ro = mk(S->cam_pos[0], S->cam_pos[1], S->cam_pos[2]);
-                    if (S->dirs) {
+                    capture = (S->dbg_hit || S->dbg_shadow) && frame == S->frame0 + S->frames - 1u;
+                    // bounce 0 from the first-bounce cache needs no primary
+                    // direction unless the shadow ray is traced (too many lights)
+                    const bool cached = S->first && !capture && S->max_bounces > 0;
+                    if (cached && S->n_light <= kFirstMaxLights) {
+                        rd = mk(0.0f, 0.0f, 0.0f);
+                    } else if (S->dirs) {
                         const float4 dv = S->dirs[px];
                         rd = mk(dv.x, dv.y, dv.z);
                     } else {
@@ -274,7 +280,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                     seed = px;   // x + y * W
                     seed *= frame;
                     bounce = 0;
-                    capture = (S->dbg_hit || S->dbg_shadow) && frame == S->frame0 + S->frames - 1u;
                     if (capture) {
                         for (uint32_t i = 0; i < S->dbg_bounces; ++i) {
                             if (S->dbg_hit) S->dbg_hit[(size_t)px * S->dbg_bounces + i] = -2;
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         }
                     }
                     state = kCont;
-                    if (S->first && !capture && S->max_bounces > 0) {
+                    if (cached) {
                         // bounce 0 from the first-bounce cache; the lane then enters
                         // the trace loop at its first continuation ray, or ends the
                         // path and takes another job
