@@ -28,6 +28,10 @@ def mixed_scenes():
     sph, mat, lig = random_scene(rng, 12, 3, 6.0, (0.0, 3.0), (8.0, 12.0))
     out.append(FlatScene(sph, mat, lig, Settings(True, False, 0), Camera.looking_at((0, 3, 12), width=20, height=10),
                          spp=2, name="zero-bounce"))
+    # no sphere, several frames: a first-bounce cache whose hit list is empty,
+    # so the scene has no path job at all inside a launch shared with others
+    out.append(FlatScene(make_spheres(0), make_materials(0), make_lights(2), Settings(True, True, 4),
+                         Camera(width=24, height=16), spp=3, name="sky-only"))
     return out
 
 

@@ -59,7 +59,7 @@ def test_server_staggered_arrivals_and_budget():
         lights = len(s.lights)
         return 68 + (16 * lights if 1 <= lights <= 32 else 0) if s.spp >= 2 else 0
     budget = max(s.camera.width * s.camera.height * (52 + cache(s)) for s in scenes) + 4096 * 64 + (1 << 17)
-    res, st = serve_all(scenes, delays=[0.0, 0.01, 0.0, 0.02, 0.0, 0.01], frames_per_tick=1,
+    res, st = serve_all(scenes, delays=[0.0, 0.01, 0.0, 0.02, 0.0, 0.01, 0.0], frames_per_tick=1,
                         hbm_budget_bytes=budget)
     assert st["hbm_high_water"] <= budget
     for (img, acc), (rimg, racc), s in zip(res, ref, scenes):
