@@ -26,9 +26,18 @@
 // a wave on nearby tree nodes and makes whole waves miss the same spheres in
 // the flat loop.  Samples are stored frame-major ([frame][pixel]), so that
 // k_accumulate reads them coalesced, and the 8 lanes that run one frame of a
-// group write one whole 128-byte line together.  Bounce 0 of a path comes
-// from the per-pixel first-bounce cache (k_first_hit) when the launch has
-// one: the lane enters the trace loop at its first continuation ray.  Spheres and the scene table
+// group write one whole 128-byte line together.
+//
+// First-bounce cache: a pixel's primary ray, the shading at its hit and the
+// shadow ray towards each light are the same in every frame.  When the
+// launch has the cache (k_first_hit, once per camera and scene), bounce 0
+// of a path is done where the job is handed out -- the cached term of the
+// frame's light if it reaches the hit, roulette, the next direction -- and
+// the lane enters the trace loop at its first continuation ray or ends the
+// path at once.  With the cache's hit list, jobs cover only the pixels
+// whose primary ray hits; k_accumulate adds the others' constant sample.
+//
+// Spheres and the scene table
 // are read with wave-uniform addresses through the constant address space,
 // which the compiler lowers to scalar loads (two spheres per s_load_dwordx8,
 // used directly as VALU operands).  A wave works on one scene at a time and

@@ -5,12 +5,12 @@
 // One launch renders frames [frame0, frame0 + frames) of every scene in its
 // table in two kernels.  The path kernel treats every (frame, pixel) path as
 // an independent job; a scene's jobs come in groups of kJobGroupPx
-// neighbouring pixels times the launch's frames (job_pixel_frame).  Waves take blocks of `chunk_px`
-// consecutive jobs of one scene from a global counter (scene s owns chunk
-// ids [chunk_begin, chunk_begin + n_chunks), n_chunks = ceil(pixels * frames
-// / chunk_px)) and hand the block's paths to their lanes one by one; a lane
-// whose path ends stores its
-// sample in the launch's sample buffer and takes the next job, so every
+// neighbouring pixels times the launch's frames (job_pixel_frame), over all
+// of its pixels or only those of its hit list.  A wave's first block is its
+// grid index, later ones come from a global counter (scene s owns chunk ids
+// [chunk_begin, chunk_begin + n_chunks), n_chunks = ceil(jobs / chunk_px));
+// it hands the block's paths to its lanes one by one; a lane whose path ends
+// stores its sample in the launch's sample buffer and takes the next job, so every
 // closest-hit loop runs with (nearly) all 64 lanes busy and no wave waits for
 // another's work.  The accumulate kernel then adds each pixel's samples to
 // its accumulation strictly in frame order and writes the RGBA8 value, which
