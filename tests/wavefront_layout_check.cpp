@@ -1,7 +1,8 @@
 // Host-side checks of the persistent kernel's launch layout
 // (ataraxia_amd/csrc/wavefront_host.h): chunk ids, pixel and sample offsets,
 // launch totals, the 32-bit guard, the flat-loop test of a table, the job
-// order (job_pixel_frame) and the chunk size chosen per launch (auto_chunk_px).
+// order (job_pixel_frame), the chunk size chosen per launch (auto_chunk_px),
+// and the first-bounce cache's layout and hit lists.
 // Built host-only (hipcc --offload-host-only) by tests/test_wavefront_host.py.
 #include <cstdio>
 #include <vector>
@@ -106,6 +107,54 @@ int main()
     for (uint64_t jobs : {uint64_t(1), uint64_t(1000), f1080, 64 * f1080, uint64_t(1) << 31}) {
         const uint32_t c = auto_chunk_px(jobs, 1536);
         CHECK(c % 64 == 0 && c >= 64 && c <= kMaxChunkPx);
+    }
+
+    // first-bounce cache layout: records, light terms, hit list and its
+    // length lie one after another, 16-byte aligned, inside first_cache_bytes
+    for (size_t npx : {size_t(1), size_t(7), size_t(1920) * 1080, size_t(4097)})
+        for (uint32_t lights : {0u, 1u, 8u, 32u, 33u}) {
+            char* b0 = reinterpret_cast<char*>(uintptr_t(1) << 24);
+            const FirstCacheView v = first_cache_view(b0, npx, lights);
+            const char* rec = reinterpret_cast<const char*>(v.rec);
+            const char* lt = reinterpret_cast<const char*>(v.light);
+            const char* hits = reinterpret_cast<const char*>(v.hits);
+            const char* nh = reinterpret_cast<const char*>(v.n_hits);
+            CHECK(rec == b0);
+            CHECK((lights >= 1 && lights <= 32) == (lt != nullptr));
+            if (lt) CHECK(lt == rec + npx * 64 && hits == lt + npx * lights * 16);
+            else CHECK(hits == rec + npx * 64);
+            CHECK(nh >= hits + npx * 4 && (uintptr_t)nh % 16 == 0 && (uintptr_t)hits % 16 == 0);
+            CHECK(nh + 4 <= b0 + first_cache_bytes(npx, lights));
+        }
+
+    // hit lists: only with a cache, at least one bounce and no debug capture;
+    // the skipped primary rays are the missing pixels' in every frame
+    {
+        char* buf = reinterpret_cast<char*>(uintptr_t(1) << 24);
+        WfScene w = scene(100, 50, 16, 3);
+        w.n_light = 8;
+        w.max_bounces = 4;
+        use_hit_list(w, buf, 1234);
+        CHECK(w.hit_px == nullptr);                       // no cache
+        set_first_cache(w, buf);
+        use_hit_list(w, buf, 1234);
+        CHECK(w.hit_px == first_cache_view(buf, 5000, 8).hits && w.n_hit == 1234);
+        CHECK(scene_jobs(w) == 1234ull * 16 && skipped_primaries(&w, 1) == (5000ull - 1234) * 16);
+        WfScene z = w;
+        z.hit_px = nullptr;
+        z.max_bounces = 0;
+        use_hit_list(z, buf, 1234);
+        CHECK(z.hit_px == nullptr && scene_jobs(z) == 5000ull * 16 && skipped_primaries(&z, 1) == 0);
+        WfScene d = w;
+        d.hit_px = nullptr;
+        d.dbg_hit = reinterpret_cast<int32_t*>(buf);
+        use_hit_list(d, buf, 1234);
+        CHECK(d.hit_px == nullptr);
+        std::vector<WfScene> two = {w, z};
+        CHECK(launch_jobs(two.data(), 2) == 1234ull * 16 + 5000ull * 16);
+        CHECK(layout_launch(two.data(), 2, 512, base, &p));
+        CHECK(two[0].n_chunks == (1234u * 16 + 511) / 512 && two[1].chunk_begin == two[0].n_chunks);
+        CHECK(two[1].samples == base + 5000 * 16);   // samples stay per pixel
     }
 
     if (failures == 0) std::printf("wavefront layout: all checks passed\n");
