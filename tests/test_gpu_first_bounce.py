@@ -174,3 +174,31 @@ def test_cache_follows_camera_and_scene():
     for x, y in zip(out[ATX_KERNEL_PLAIN], out[ATX_KERNEL_WAVEFRONT]):
         assert np.array_equal(x, y)
     assert not np.array_equal(out[ATX_KERNEL_WAVEFRONT][0], out[ATX_KERNEL_WAVEFRONT][1])
+
+
+def test_cache_kept_across_accel_and_settings_changes():
+    """The renderer keeps its cache while camera and scene stay; switching
+    tree/flat loop, sky light and bounce depth between calls must give the
+    plain kernel's accumulation after every call."""
+    from ataraxia_amd import ATX_ACCEL_BVH, ATX_ACCEL_NONE
+    sc = CASES["plain-cpu"]
+    cam = sc.camera
+    steps = [(ATX_ACCEL_BVH, Settings(True, False, 5), 2), (ATX_ACCEL_NONE, Settings(True, False, 5), 1),
+             (ATX_ACCEL_BVH, Settings(True, True, 5), 2), (ATX_ACCEL_BVH, Settings(True, True, 0), 1),
+             (ATX_ACCEL_NONE, Settings(True, False, 3), 3)]
+    out = {}
+    for kernel in (ATX_KERNEL_PLAIN, ATX_KERNEL_WAVEFRONT):
+        accs = []
+        with Renderer() as r:
+            r.set_kernel(kernel)
+            r.on_resize(cam.width, cam.height)
+            r.set_settings(sc.settings)
+            r.set_scene(sc)
+            for accel, st, frames in steps:
+                r.set_accel(accel)
+                r.set_settings(st)
+                r.render_frames(cam, frames)
+                accs.append(r.get_accumulation().copy())
+        out[kernel] = accs
+    for k, (x, y) in enumerate(zip(out[ATX_KERNEL_PLAIN], out[ATX_KERNEL_WAVEFRONT])):
+        assert np.array_equal(x, y), k
