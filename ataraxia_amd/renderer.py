@@ -120,6 +120,11 @@ class Renderer:
         return acc.reshape(self.height, self.width, 4)
 
     def stats(self) -> dict:
+        """Counters of the last render call: samples, rays (every ray the
+        reference traces) and reused_rays (those answered from the
+        first-bounce cache), sphere and box tests, exact retraces, launches,
+        and HIP-event times of the call (gpu_ms) and of the path kernel
+        (path_ms)."""
         s = _lib.RenderStats()
         check(lib.atx_renderer_get_stats(self._h, C.byref(s)))
         return s.as_dict()
