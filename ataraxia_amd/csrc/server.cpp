@@ -5,7 +5,8 @@
 // the admitted ones fill less than tick_mpaths million paths per launch (so
 // the device stays full without every queued request sharing it) and their
 // device buffers fit the HBM budget (geometry, accumulation, optional Kahan
-// compensation, RGBA8 image, primary directions, the tick's samples).  The
+// compensation, RGBA8 image, primary directions, first-bounce cache, the
+// tick's samples).  The
 // next request is the one with the highest response ratio, (waiting time +
 // estimated render time) / estimated render time, with the render time
 // estimated from the throughput measured so far: short requests go first and
@@ -14,10 +15,12 @@
 // wavefront kernel per tick (frames_per_tick frames per request, each request
 // at its own frame index).  Requests that arrive while a launch runs join at
 // the next launch: the worker enqueues a tick and admits the next tick's
-// requests while it runs.  Requests that reach their spp go to a retire
-// thread, which copies their results to host memory and marks them done; the
-// worker releases their device buffers once the running tick has ended (a
-// release waits for the device), which makes room for the next admissions.
+// requests while it runs (their primary rays and first-bounce caches are
+// computed on a second stream, beside the tick).  Requests that reach their
+// spp go to a retire thread, which copies their results to host memory and
+// marks them done; the worker then returns their device blocks to a pool
+// that later admissions take from (a hipFree would wait for the device),
+// which makes room for the next admissions.
 // Per-request timestamps (submit, admit, done) give the latency
 // distribution.
 #include <algorithm>
