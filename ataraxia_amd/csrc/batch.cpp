@@ -9,7 +9,9 @@
 // a kernel that zeroes the counters) are captured once into a HIP graph and
 // replayed by every render call.  Chunk size, frames per launch and grid size
 // come from atx_batch_set_tuning or from the autotuner (autotune.cpp), whose
-// choice is cached per device and workload.
+// choice is cached per device and workload.  Every scene of two or more
+// frames builds its primary directions and first-bounce cache once, when the
+// batch is first prepared; launches then run paths only for its hit pixels.
 #include <algorithm>
 #include <memory>
 #include <new>

@@ -8,7 +8,11 @@
 //    (Camera.cpp:197-220);
 //  * no per-frame occupancy/device-property queries or full-device syncs;
 //    work is queued on the handle's own stream and `render` waits on it;
-//  * the RGBA8 image stays on the device until atx_renderer_get_image.
+//  * the RGBA8 image stays on the device until atx_renderer_get_image;
+//  * the persistent kernel (the default) renders up to 256 frames per
+//    launch, and keeps per camera and scene the primary directions and the
+//    first-bounce cache (built for calls of two or more frames, or once the
+//    camera has not moved since the previous call).
 #include <algorithm>
 #include <new>
 #include <stdlib.h>
