@@ -51,12 +51,18 @@ public:
     DevPool(const DevPool&) = delete;
     DevPool& operator=(const DevPool&) = delete;
     void set_limit(size_t limit) { std::lock_guard<std::mutex> lk(mu_); limit_ = limit; }
-    // A pooled block of capacity in [bytes, 2 * bytes + 64 KiB], or nullptr.
+    // Capacity beyond the bytes asked for that take() may still hand out, in
+    // total over the next calls (SIZE_MAX: no limit).  An owner that meters
+    // device bytes against a budget sets it to what the budget has left.
+    void set_slack(size_t slack) { std::lock_guard<std::mutex> lk(mu_); slack_ = slack; }
+    // A pooled block of capacity in [bytes, 2 * bytes + 64 KiB] whose excess
+    // over `bytes` fits the slack, or nullptr.
     void* take(size_t bytes, size_t* cap)
     {
         std::lock_guard<std::mutex> lk(mu_);
         auto it = free_.lower_bound(bytes);
-        if (it == free_.end() || it->first > 2 * bytes + (64u << 10)) return nullptr;
+        if (it == free_.end() || it->first > 2 * bytes + (64u << 10) || it->first - bytes > slack_) return nullptr;
+        if (slack_ != SIZE_MAX) slack_ -= it->first - bytes;
         void* p = it->second;
         *cap = it->first;
         pooled_ -= it->first;
@@ -90,7 +96,7 @@ public:
 private:
     mutable std::mutex mu_;
     std::multimap<size_t, void*> free_;   // capacity -> block
-    size_t pooled_ = 0, limit_ = 0;
+    size_t pooled_ = 0, limit_ = 0, slack_ = SIZE_MAX;
 };
 
 // Move-only owner of one device allocation (the reference's CudaBuffer has a

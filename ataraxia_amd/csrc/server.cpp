@@ -126,6 +126,18 @@ static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_ti
            (r.spp >= 2 ? first_cache_bytes(npx, (uint32_t)r.lights.size()) : 0);
 }
 
+// Device bytes an admitted request actually holds: its blocks' capacities
+// (a pooled block may be up to twice the size asked for) and its share of
+// the tick's sample buffer.  Admission checks request_bytes against the
+// budget, then charges this.
+static size_t held_bytes(const Request& r, uint32_t frames_per_tick)
+{
+    const size_t npx = (size_t)r.cam.width * r.cam.height;
+    const size_t f = std::min<size_t>(frames_per_tick, r.spp);
+    return r.geo.bytes() + r.accum.bytes() + r.comp.bytes() + r.image.bytes() + r.dirs.bytes() + r.first.bytes() +
+           npx * f * 16;
+}
+
 // A launch's paths must stay below 2^31 (the kernel's 32-bit job indices).
 constexpr uint64_t kMaxTickPaths = 1ull << 31;
 
@@ -285,8 +297,21 @@ static void admit_queued(Server* s)
     if (newly.empty()) return;
     const auto t0 = std::chrono::steady_clock::now();
     for (auto& r : newly) {
-        if (admit(s, *r)) {
+        {
+            // pooled blocks larger than asked for may use only what the
+            // budget has left, so the bytes actually held stay within it
             std::lock_guard<std::mutex> lk(s->mu);
+            s->pool.set_slack(s->cfg.hbm_budget_bytes > s->stats.hbm_in_use
+                                  ? (size_t)(s->cfg.hbm_budget_bytes - s->stats.hbm_in_use) : 0);
+        }
+        const bool admitted = admit(s, *r);
+        s->pool.set_slack(SIZE_MAX);
+        if (admitted) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            const size_t held = held_bytes(*r, s->cfg.frames_per_tick);
+            s->stats.hbm_in_use = s->stats.hbm_in_use - r->device_bytes + held;
+            s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
+            r->device_bytes = held;
             r->state = ATX_REQ_RUNNING;
             r->t_admit = now_s(s);
             s->stats.admitted++;

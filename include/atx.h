@@ -241,7 +241,8 @@ typedef struct atx_server_stats {
     uint64_t queued, active;
     uint64_t samples;              /* path samples of completed requests        */
     uint64_t ticks;                /* launches issued                           */
-    uint64_t hbm_in_use;           /* bytes of the budget currently allocated   */
+    uint64_t hbm_in_use;           /* device bytes admitted requests hold (block
+                                      capacities + share of the tick's samples) */
     uint64_t hbm_high_water;       /* max of hbm_in_use                         */
     uint64_t device_used_high_water; /* max device memory in use (hipMemGetInfo) */
     double gpu_busy_ms;            /* HIP-event time of all launches            */
