@@ -1,6 +1,8 @@
 """Host-side checks of the persistent kernel's launch layout (no GPU): chunk
 ids, pixel and sample offsets and the 32-bit guard of
-ataraxia_amd/csrc/wavefront_host.h, compiled host-only by hipcc."""
+ataraxia_amd/csrc/wavefront_host.h, the job order, the first-bounce cache
+layout, and the device-block pool's hand-out rules, compiled host-only by
+hipcc."""
 import os
 import shutil
 import subprocess

@@ -2,7 +2,8 @@
 // (ataraxia_amd/csrc/wavefront_host.h): chunk ids, pixel and sample offsets,
 // launch totals, the 32-bit guard, the flat-loop test of a table, the job
 // order (job_pixel_frame), the chunk size chosen per launch (auto_chunk_px),
-// and the first-bounce cache's layout and hit lists.
+// the first-bounce cache's layout and hit lists, and the hand-out rules of
+// the device-block pool (host_common.h: DevPool).
 // Built host-only (hipcc --offload-host-only) by tests/test_wavefront_host.py.
 #include <cstdio>
 #include <vector>
@@ -155,6 +156,27 @@ int main()
         CHECK(layout_launch(two.data(), 2, 512, base, &p));
         CHECK(two[0].n_chunks == (1234u * 16 + 511) / 512 && two[1].chunk_begin == two[0].n_chunks);
         CHECK(two[1].samples == base + 5000 * 16);   // samples stay per pixel
+    }
+
+    // device-block pool: a block goes to a request of at least half its size
+    // (+64 KiB) whose excess fits the slack.  Fake addresses; every block is
+    // taken back before the pool is destroyed, so nothing is freed.
+    {
+        DevPool pool(size_t(1) << 40);
+        char* b = reinterpret_cast<char*>(uintptr_t(1) << 30);
+        pool.give(b, 1000);
+        pool.give(b + 4096, 300000);
+        size_t cap = 0;
+        CHECK(pool.pooled() == 301000);
+        CHECK(pool.take(400000, &cap) == nullptr);                      // no block that large
+        CHECK(pool.take(100000, &cap) == nullptr);                      // 300000 > 2 * 100000 + 64 KiB
+        pool.set_slack(100000);
+        CHECK(pool.take(190000, &cap) == nullptr);                      // excess 110000 > slack
+        CHECK(pool.take(200000, &cap) == b + 4096 && cap == 300000);    // excess 100000 = slack
+        CHECK(pool.take(900, &cap) == nullptr);                         // slack used up
+        pool.set_slack(SIZE_MAX);
+        CHECK(pool.take(900, &cap) == b && cap == 1000);
+        CHECK(pool.pooled() == 0);
     }
 
     if (failures == 0) std::printf("wavefront layout: all checks passed\n");
