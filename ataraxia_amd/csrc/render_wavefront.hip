@@ -489,13 +489,22 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
     float4 acc = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.accum[px];
     if (S.kahan) {
         float4 cp = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.comp[px];
-        for (uint32_t f = 0; f < S.frames; ++f) {
-            const float4 c = sky ? sky_c : S.samples[(size_t)f * npx + px];
+        const auto add = [&](const float4 c) {
             float yv, t;
             yv = c.x - cp.x; t = acc.x + yv; cp.x = (t - acc.x) - yv; acc.x = t;
             yv = c.y - cp.y; t = acc.y + yv; cp.y = (t - acc.y) - yv; acc.y = t;
             yv = c.z - cp.z; t = acc.z + yv; cp.z = (t - acc.z) - yv; acc.z = t;
             yv = c.w - cp.w; t = acc.w + yv; cp.w = (t - acc.w) - yv; acc.w = t;
+        };
+        // two loops: a select between sky_c and a loaded sample becomes a
+        // select of addresses, and sky_c then lives in scratch memory
+        if (sky) {
+            for (uint32_t f = 0; f < S.frames; ++f)
+                add(sky_c);
+        } else {
+            const ATX_GLOBAL float4* sp = (const ATX_GLOBAL float4*)(uintptr_t)S.samples + px;
+            for (uint32_t f = 0; f < S.frames; ++f)
+                add(sp[(size_t)f * npx]);
         }
         S.comp[px] = cp;
     } else if (sky) {
@@ -507,7 +516,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
         }
     } else {
         // four frames' loads in flight ahead of their additions (in frame order)
-        const float4* sp = S.samples + px;
+        const ATX_GLOBAL float4* sp = (const ATX_GLOBAL float4*)(uintptr_t)S.samples + px;
         uint32_t f = 0;
         for (; f + 4u <= S.frames; f += 4u) {
             float4 c[4];
