@@ -36,7 +36,8 @@ class OraCamera(C.Structure):
 
 
 class OraPathDebug(C.Structure):
-    _fields_ = [("hit_ids", _P), ("shadow_ids", _P), ("rays", _U64), ("bounces", C.c_int32)]
+    _fields_ = [("hit_ids", _P), ("shadow_ids", _P), ("rays", _U64), ("bounces", C.c_int32),
+                ("seeds", _P), ("light_ids", _P), ("rr", _P), ("dirs", _P)]
 
 
 _PROTOS = {
@@ -146,6 +147,25 @@ def per_pixel(x, y, width, direction, cam_pos, scene: Scene, frame, sett: OraSet
     o = np.ascontiguousarray(cam_pos, dtype=np.float32)
     lib.ora_per_pixel(x, y, width, _p(d), _p(o), C.byref(scene.c), frame, C.byref(sett), _p(out), C.byref(dbg))
     return out, hit, sh, dbg.rays
+
+
+def per_pixel_trace(x, y, width, direction, cam_pos, scene: Scene, frame, sett: OraSettings) -> dict:
+    """One path with its per-bounce log: hit and shadow ids, the seed after
+    `seed += i`, the light index, roulette p and u, and the traced direction."""
+    nb = max(1, sett.max_bounces)
+    log = {"hit": np.full(nb, -2, np.int32), "shadow": np.full(nb, -2, np.int32),
+           "seed": np.zeros(nb, np.uint32), "light": np.full(nb, -1, np.int32),
+           "rr": np.full((nb, 2), np.nan, np.float32), "dir": np.full((nb, 3), np.nan, np.float32)}
+    dbg = OraPathDebug(_p(log["hit"]), _p(log["shadow"]), 0, 0, _p(log["seed"]), _p(log["light"]),
+                       _p(log["rr"]), _p(log["dir"]))
+    out = np.zeros(4, np.float32)
+    d = np.ascontiguousarray(direction, dtype=np.float32)
+    o = np.ascontiguousarray(cam_pos, dtype=np.float32)
+    lib.ora_per_pixel(x, y, width, _p(d), _p(o), C.byref(scene.c), frame, C.byref(sett), _p(out), C.byref(dbg))
+    log["rgba"] = out
+    log["rays"] = dbg.rays
+    log["bounces"] = dbg.bounces
+    return log
 
 
 def render_frame(cam: OraCamera, scene: Scene, sett: OraSettings, frame: int, accum=None, comp=None,

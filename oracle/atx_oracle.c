@@ -32,10 +32,22 @@
  *   per_pixel             sky-only scene == (0.6,0.7,0.9); emissive sphere with one
  *                         bounce == emission; one diffuse sphere + one light with
  *                         one bounce == float64 evaluation of the path by hand;
- *                         occluded light == 0; no lights; zero bounces == 0
+ *                         occluded light == 0; no lights; zero bounces == 0;
+ *                         multi-bounce paths (up to 6 bounces, diffuse and
+ *                         metallic, two lights, sky) against a separate float64
+ *                         restatement with Python-integer PCG (tests/_pathref.py):
+ *                         per-bounce hit and shadow ids, seeds after `seed += i`,
+ *                         light indices and roulette draws exact, p, directions
+ *                         and radiance to rounding; closed forms: albedo 1 under
+ *                         the sky == sky exactly, albedo 0.5 == sky*0.5/p or 0 by
+ *                         the hash-chain draw, first continuation cos(theta) from
+ *                         u1 for GGX and cosine
  *   render_frame          accumulation mean, RGBA8 truncation table, Kahan vs
  *                         naive vs float64 sums, golden images in tests/golden
- *   flatten               sample-scene-data/scene.json known answers
+ *   flatten               sample-scene-data/scene.json known answers; closed
+ *                         forms of quaternion order ([x,y,z,w] in the file),
+ *                         90-degree rotations, T*R*S order, parent*local order
+ *                         and the mean-column-length radius scale
  * No oracle function is left without one of these checks.
  */
 #include "atx_oracle.h"
@@ -470,6 +482,10 @@ void ora_per_pixel(uint32_t x, uint32_t y, uint32_t width, const float dir[3],
         for (int i = 0; i < bounces; ++i) {
             if (dbg->hit_ids) dbg->hit_ids[i] = -2;
             if (dbg->shadow_ids) dbg->shadow_ids[i] = -2;
+            if (dbg->seeds) dbg->seeds[i] = 0u;
+            if (dbg->light_ids) dbg->light_ids[i] = -1;
+            if (dbg->rr) dbg->rr[2 * i] = dbg->rr[2 * i + 1] = NAN;
+            if (dbg->dirs) dbg->dirs[3 * i] = dbg->dirs[3 * i + 1] = dbg->dirs[3 * i + 2] = NAN;
         }
     }
     for (int i = 0; i < bounces; i++) {
@@ -479,6 +495,8 @@ void ora_per_pixel(uint32_t x, uint32_t y, uint32_t width, const float dir[3],
             dbg->rays += 1;
             dbg->bounces = i + 1;
             if (dbg->hit_ids) dbg->hit_ids[i] = ht.id;
+            if (dbg->seeds) dbg->seeds[i] = seed;
+            if (dbg->dirs) { dbg->dirs[3 * i] = rd.x; dbg->dirs[3 * i + 1] = rd.y; dbg->dirs[3 * i + 2] = rd.z; }
         }
         if (ht.t < 0.0f) {
             if (settings->sky_light) {
@@ -498,6 +516,7 @@ void ora_per_pixel(uint32_t x, uint32_t y, uint32_t width, const float dir[3],
 
         if (sc->num_lights > 0) {
             uint32_t light_index = (uint32_t)((uint64_t)ora_pcg_hash(seed) % sc->num_lights);
+            if (dbg && dbg->light_ids) dbg->light_ids[i] = (int32_t)light_index;
             const ora_light* light = &sc->lights[light_index];
             v3 L = v3sub(v3load(light->position), ht.pos);
             float dist2 = v3dot(L, L);
@@ -528,7 +547,9 @@ void ora_per_pixel(uint32_t x, uint32_t y, uint32_t width, const float dir[3],
         ro = v3add(ht.pos, v3scale(ht.normal, 0.0001f));
 
         float p = gmax(0.1f, gmin(1.0f, v3len(throughput)));
-        if (ora_pcg_float(&seed) > p)
+        float u = ora_pcg_float(&seed);
+        if (dbg && dbg->rr) { dbg->rr[2 * i] = p; dbg->rr[2 * i + 1] = u; }
+        if (u > p)
             break;
         throughput = v3divs(throughput, p);
 
@@ -574,7 +595,7 @@ void ora_render_frame(const ora_camera* cam, const ora_scene* sc, const ora_sett
         for (uint32_t x = 0; x < W; ++x) {
             float d[3], c[4];
             ora_ray_direction(ip, iv, x, y, W, cam->height, d);
-            ora_path_debug dbg = {0, 0, 0, 0};
+            ora_path_debug dbg = {0};
             ora_per_pixel(x, y, W, d, cam->position, sc, frame_index, settings, c, rays ? &dbg : 0);
             nrays += dbg.rays;
             size_t px = (size_t)x + (size_t)y * W;
@@ -620,7 +641,9 @@ void ora_render_frame_debug(const ora_camera* cam, const ora_scene* sc, const or
             float d[3], c[4];
             size_t px = (size_t)x + (size_t)y * W;
             ora_ray_direction(ip, iv, x, y, W, cam->height, d);
-            ora_path_debug dbg = {hb, sb, 0, 0};
+            ora_path_debug dbg = {0};
+            dbg.hit_ids = hb;
+            dbg.shadow_ids = sb;
             ora_per_pixel(x, y, W, d, cam->position, sc, frame_index, settings, c, &dbg);
             if (radiance) memcpy(radiance + 4 * px, c, sizeof(c));
             for (uint32_t b = 0; b < max_bounces; ++b) {

@@ -81,12 +81,24 @@ void ora_sample_ggx(const float N[3], float roughness, uint32_t* seed, float out
 
 /* Per-path debug record.  hit_ids[b] is the closest-hit index of the bounce-b
  * continuation ray (primary for b == 0) and shadow_ids[b] that of its shadow ray
- * (-1 = miss, -2 = not traced).  Arrays hold max_bounces entries.            */
+ * (-1 = miss, -2 = not traced).  Arrays hold max_bounces entries (rr: 2 per
+ * bounce, dirs: 3 per bounce).  Every array pointer may be NULL.  The last
+ * four expose the random stream and the continuation decisions per bounce so
+ * that an independent restatement can check them:
+ *   seeds[b]     the u32 seed right after `seed += b` (Renderer.cu:306)
+ *   light_ids[b] PcgHash(seed) % numLights, -1 without lights (:340)
+ *   rr[2b..2b+1] Russian roulette p and u (:375-376); NaN if not reached
+ *   dirs[3b..]   direction of the ray traced at bounce b (primary at b = 0);
+ *                NaN if not reached                                       */
 typedef struct {
     int32_t* hit_ids;
     int32_t* shadow_ids;
     uint64_t rays;        /* rays traced by this path (continuation + shadow) */
     int32_t  bounces;     /* bounce iterations entered                       */
+    uint32_t* seeds;
+    int32_t*  light_ids;
+    float*    rr;
+    float*    dirs;
 } ora_path_debug;
 
 void ora_per_pixel(uint32_t x, uint32_t y, uint32_t width, const float dir[3],

@@ -54,9 +54,9 @@ def oracle_flatten(nodes):
 
     def visit(i, parent_m):
         _, pos, q, s, sph = nodes[i]
+        pos, q, s = (np.ascontiguousarray(v, np.float32) for v in (pos, q, s))
         local = np.zeros(16, np.float32)
-        O.lib.ora_node_local_transform(np.float32(pos).ctypes.data, np.float32(q).ctypes.data,
-                                       np.float32(s).ctypes.data, local.ctypes.data)
+        O.lib.ora_node_local_transform(pos.ctypes.data, q.ctypes.data, s.ctypes.data, local.ctypes.data)
         G = np.zeros(16, np.float32)
         O.lib.ora_mat4_mul(parent_m.ctypes.data, local.ctypes.data, G.ctypes.data)
         for c, r, m in sph:
@@ -139,3 +139,106 @@ def test_write_ppm(tmp_path):
     px = data[len(b"P6\n2 2\n255\n"):]
     # flipped: the last row first; bytes are r, g, b
     assert list(px) == [0, 0, 255, 0x56, 0x34, 0x12, 255, 0, 0, 0, 255, 0]
+
+
+# ---- closed-form pins of flattening (SceneNode.cpp:46, Renderer.cu:78-86,
+# Utils.cpp:145).  Each case names the answer a plausible mistake would give.
+
+S2 = math.sqrt(0.5)
+
+
+def _scene_json(root, children=()):
+    """A scene.json text with one root node and nested children; each node is
+    (position, rotation as the file stores it [x, y, z, w], scale, spheres)."""
+    import json
+
+    def node(name, spec, kids):
+        pos, rot, scl, sph = spec
+        d = {"name": name, "transformation": {"position": list(pos), "rotation": list(rot), "scale": list(scl)}}
+        if sph:
+            d["spheres"] = [{"center": list(c), "radius": r, "materialIndex": 0} for c, r in sph]
+        if kids:
+            d["children"] = [node(f"{name}.{i}", k[0], k[1]) for i, k in enumerate(kids)]
+        return d
+
+    doc = {"camera": {"direction": [0, 0, -1], "fov": 45.0, "position": [0, 0, 5]},
+           "lights": [], "materials": [{"F0": [0.04] * 3, "albedo": [1, 1, 1], "emissionColor": [0, 0, 0],
+                                        "emissionIntensity": 0.0, "metallic": 0.0, "roughness": 0.5}],
+           "sceneGraph": node("Root", root, list(children)),
+           "settings": {"accumulation": True, "maxBounces": 4, "skyLight": False}}
+    return json.dumps(doc)
+
+
+def _oracle_node_flatten(parent_m, pos, rot_xyzw, scl, sph):
+    # the arrays are bound to names so that they outlive the calls reading them
+    pos, rot, scl, par = (np.ascontiguousarray(v, np.float32) for v in (pos, rot_xyzw, scl, parent_m))
+    local = np.zeros(16, np.float32)
+    O.lib.ora_node_local_transform(pos.ctypes.data, rot.ctypes.data, scl.ctypes.data, local.ctypes.data)
+    G = np.zeros(16, np.float32)
+    O.lib.ora_mat4_mul(par.ctypes.data, local.ctypes.data, G.ctypes.data)
+    out = []
+    for c, r in sph:
+        a = np.zeros(1, _lib.SPHERE_DTYPE)
+        a["center"][0], a["radius"][0] = c, r
+        o = np.zeros(1, _lib.SPHERE_DTYPE)
+        O.lib.ora_flatten_sphere(G.ctypes.data, a.ctypes.data, o.ctypes.data)
+        out.append(o[0])
+    return G, out
+
+
+def test_flatten_quaternion_order_and_rotation():
+    """scene.json stores [x, y, z, w] and glm::quat takes (w, x, y, z): 90
+    degrees about z maps (1,0,0) to (0,1,0) (reading the file as w,x,y,z
+    would rotate about y instead, mapping (1,0,0) to (0,0,-1)); 90 degrees
+    about x maps (0,1,0) to (0,0,1)."""
+    I = np.eye(4, dtype=np.float32).ravel()
+    for rot, c, exp in (([0, 0, S2, S2], (1, 0, 0), (0, 1, 0)), ([S2, 0, 0, S2], (0, 1, 0), (0, 0, 1)),
+                        ([0, S2, 0, S2], (0, 0, 1), (1, 0, 0)), ([0, 0, 1, 0], (1, 2, 0), (-1, -2, 0))):
+        g = SceneGraph.parse_json(_scene_json(((0, 0, 0), rot, (1, 1, 1), [(c, 1.0)])))
+        got = g.flatten()[0]
+        np.testing.assert_allclose(got["center"], exp, atol=1e-6)
+        assert abs(float(got["radius"]) - 1.0) < 1e-6
+        _, (o,) = _oracle_node_flatten(I, (0, 0, 0), rot, (1, 1, 1), [(c, 1.0)])
+        np.testing.assert_allclose(o["center"], exp, atol=1e-6)
+        assert np.array_equal(o["center"], got["center"])
+
+
+def test_flatten_scale_and_trs_order():
+    """Radius scales by the mean of the transform's column lengths: scale
+    (1, 2, 3) doubles it (with or without a rotation); the node transform is
+    T * R * S, so position (1,0,0), 90 degrees about z and scale 2 take
+    (1,0,0) to (1,2,0) (S * R * T would give (0,4,0))."""
+    I = np.eye(4, dtype=np.float32).ravel()
+    cases = [(((0, 0, 0), [0, 0, 0, 1], (1, 2, 3)), (1, 1, 1), 0.5, (1, 2, 3), 1.0),
+             (((0, 0, 0), [0, 0, S2, S2], (1, 2, 3)), (0, 1, 0), 0.5, (-2, 0, 0), 1.0),
+             (((1, 0, 0), [0, 0, S2, S2], (2, 2, 2)), (1, 0, 0), 1.5, (1, 2, 0), 3.0)]
+    for (pos, rot, scl), c, r, exp_c, exp_r in cases:
+        g = SceneGraph.parse_json(_scene_json((pos, rot, scl, [(c, r)])))
+        got = g.flatten()[0]
+        np.testing.assert_allclose(got["center"], exp_c, atol=2e-6)
+        assert abs(float(got["radius"]) - exp_r) < 2e-6, (got["radius"], exp_r)
+        _, (o,) = _oracle_node_flatten(I, pos, rot, scl, [(c, r)])
+        assert np.array_equal(o["center"], got["center"]) and o["radius"] == got["radius"]
+    # exactly representable case: identity rotation, scale (1,2,3): radius 0.5 -> 1.0 exactly
+    g = SceneGraph.parse_json(_scene_json(((0, 0, 0), [0, 0, 0, 1], (1, 2, 3), [((1, 1, 1), 0.5)])))
+    s = g.flatten()[0]
+    assert s["radius"] == np.float32(1.0) and s["center"].tolist() == [1, 2, 3]
+
+
+def test_flatten_parent_times_child():
+    """global = parent * local: a parent at (2,0,0) rotated 90 degrees about z
+    with a child at (1,0,0) puts the child's origin at (2,1,0) (local *
+    parent would give (3,0,0)); the child's scale 3 triples its radius and
+    the parent's rotation turns the child's offset sphere centre."""
+    root = ((2, 0, 0), [0, 0, S2, S2], (1, 1, 1), [])
+    child = (((1, 0, 0), [0, 0, 0, 1], (3, 3, 3), [((0, 0, 0), 1.0), ((1, 0, 0), 0.5)]), [])
+    g = SceneGraph.parse_json(_scene_json(root, [child]))
+    s = g.flatten()
+    np.testing.assert_allclose(s["center"][0], (2, 1, 0), atol=2e-6)
+    np.testing.assert_allclose(s["center"][1], (2, 4, 0), atol=4e-6)
+    np.testing.assert_allclose(s["radius"], (3.0, 1.5), atol=4e-6)
+    I = np.eye(4, dtype=np.float32).ravel()
+    G, _ = _oracle_node_flatten(I, (2, 0, 0), [0, 0, S2, S2], (1, 1, 1), [])
+    _, outs = _oracle_node_flatten(G, (1, 0, 0), [0, 0, 0, 1], (3, 3, 3), [((0, 0, 0), 1.0), ((1, 0, 0), 0.5)])
+    for o, got in zip(outs, s):
+        assert np.array_equal(o["center"], got["center"]) and o["radius"] == got["radius"]
