@@ -105,32 +105,39 @@ int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* mate
         ATX_HIP(hipMemcpy(out->bvh_lidx.get(), T.lidx.data(), T.lidx.size() * 4, hipMemcpyHostToDevice));
         if (!T.big.empty())
             ATX_HIP(hipMemcpy(out->bvh_big.get(), T.big.data(), T.big.size() * 4, hipMemcpyHostToDevice));
-        BvhDev& b = out->bvh;
-        b.nodes = out->bvh_nodes.as<uint4>();
-        b.lsph = out->bvh_lsph.as<float4>();
-        b.lidx = out->bvh_lidx.as<int32_t>();
-        b.big = out->bvh_big.as<int32_t>();
-        b.n_nodes = T.n_nodes;
-        b.n_big = (uint32_t)T.big.size();
-        b.layout_mask = T.layouts - 1u;
-        b.cx = T.cx;
-        b.cy = T.cy;
-        b.cz = T.cz;
-        b.rb = T.rb;
-        b.r_min = T.r_min;
-        b.r_max = T.r_max;
-        b.bx = T.base[0];
-        b.by = T.base[1];
-        b.bz = T.base[2];
-        b.sx = T.scale[0];
-        b.sy = T.scale[1];
-        b.sz = T.scale[2];
-        b.isx = T.inv_scale[0];
-        b.isy = T.inv_scale[1];
-        b.isz = T.inv_scale[2];
-        b.ext = T.ext;
+        out->bvh = bvh_dev_from(T, out->bvh_nodes.get(), out->bvh_lsph.get(), out->bvh_lidx.get(),
+                                out->bvh_big.get());
     }
     return ATX_OK;
+}
+
+BvhDev bvh_dev_from(const HostBvh& T, const void* nodes, const void* lsph, const void* lidx, const void* big)
+{
+    BvhDev b{};
+    b.nodes = static_cast<const uint4*>(nodes);
+    b.lsph = static_cast<const float4*>(lsph);
+    b.lidx = static_cast<const int32_t*>(lidx);
+    b.big = static_cast<const int32_t*>(big);
+    b.n_nodes = T.n_nodes;
+    b.n_big = (uint32_t)T.big.size();
+    b.layout_mask = T.layouts - 1u;
+    b.cx = T.cx;
+    b.cy = T.cy;
+    b.cz = T.cz;
+    b.rb = T.rb;
+    b.r_min = T.r_min;
+    b.r_max = T.r_max;
+    b.bx = T.base[0];
+    b.by = T.base[1];
+    b.bz = T.base[2];
+    b.sx = T.scale[0];
+    b.sy = T.scale[1];
+    b.sz = T.scale[2];
+    b.isx = T.inv_scale[0];
+    b.isy = T.inv_scale[1];
+    b.isz = T.inv_scale[2];
+    b.ext = T.ext;
+    return b;
 }
 
 }  // namespace atx

@@ -201,6 +201,10 @@ struct DevScene {
 int upload_scene(const atx_sphere* spheres, uint64_t n, const atx_material* materials, uint64_t m,
                  const atx_light* lights, uint64_t l, DevScene* out);
 
+// Kernel view of a host-built tree whose arrays were copied to the given
+// device addresses (HostBvh::nodes, lsph, lidx and big, in that order).
+BvhDev bvh_dev_from(const HostBvh& T, const void* nodes, const void* lsph, const void* lidx, const void* big);
+
 // Device bytes a scene of this size occupies (geometry and tree).
 size_t scene_device_bytes(uint64_t n, uint64_t m, uint64_t l);
 

@@ -158,6 +158,29 @@ int atx_renderer_set_kernel(atx_renderer* r, int kind);
 /* ATX_ACCEL_BVH (default) or ATX_ACCEL_NONE; results are identical.        */
 int atx_renderer_set_accel(atx_renderer* r, int accel);
 
+/* ---- Allocation-free rendering ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): renders `frames`
+ * frames of one scene with the plain kernel, exactly as
+ * atx_renderer_render_frames does with ATX_KERNEL_PLAIN, frame indices
+ * first_frame, first_frame + 1, ... (with accumulation off the index returns
+ * to 1 after the first frame, Renderer.cu:245-248; frame 1 starts the
+ * accumulation from zero).  The caller passes device buffers of width*height
+ * entries: accum (4 floats per pixel), comp (4 floats per pixel, Kahan only,
+ * else may be NULL) and image (RGBA8).  The scene (copied from host memory),
+ * its sphere tree and the kernel's counters go into a fixed 8 MiB device
+ * array that the library holds per device for the life of the process; a
+ * scene that needs more returns ATX_ERR_INVALID.  No device or pinned host
+ * memory is allocated.  Work is queued on `stream` (a hipStream_t, NULL =
+ * the null stream) and the call returns when it has finished.  Calls on one
+ * device share the array: calls that overlap (two threads, or two streams)
+ * race.                                                                      */
+int atx_render_frames_noalloc(const atx_sphere* spheres, uint64_t num_spheres,
+                              const atx_material* materials, uint64_t num_materials,
+                              const atx_light* lights, uint64_t num_lights, const atx_settings* settings,
+                              const atx_camera* cam, uint32_t first_frame, uint32_t frames, int accel,
+                              int accumulator, float* accum, float* comp, uint32_t* image, void* stream);
+
 /* ---- Batch: many scenes, persistent launches of several frames each ------ */
 /* No reference counterpart: the reference renders one scene per Render call.
  * Scene arrays are copied at add time; every scene keeps its own accumulation

@@ -133,3 +133,29 @@ def test_workload_scenes():
         assert 5 <= s.num_spheres <= 1025 and 16 <= s.spp <= 1024
     # determinism: the same seed gives the same bytes
     assert plain_1080p_scene().spheres.tobytes() == plain_1080p_scene().spheres.tobytes()
+
+
+def test_noalloc_entry_rejects_before_touching_the_device():
+    """atx_render_frames_noalloc checks its arguments and the scene's scratch
+    size on the host: a scene beyond the 8 MiB device array, a missing
+    Kahan buffer or frame index 0 is an error, not a launch."""
+    from ataraxia_amd.scene import Settings as PySettings
+    n = 600_000                                    # 16 + 4 B per sphere: ~12 MB > 8 MiB
+    sph = np.zeros(n, _lib.SPHERE_DTYPE)
+    sph["radius"] = 1.0
+    mat = np.zeros(1, _lib.MATERIAL_DTYPE)
+    st = PySettings(True, False, 4).to_c()
+    cam = Camera(width=8, height=8).to_c()
+    fake = C.c_void_p(0x1000)                      # never dereferenced: the call fails first
+
+    def call(spheres, first_frame=1, accum_kind=_lib.ATX_ACCUM_NAIVE, comp=None):
+        return _lib.lib.atx_render_frames_noalloc(
+            _lib.ptr(spheres), len(spheres), _lib.ptr(mat), 1, None, 0, C.byref(st), C.byref(cam), first_frame, 2,
+            _lib.ATX_ACCEL_NONE, accum_kind, fake, comp, fake, None)
+
+    assert call(sph) == _lib.ATX_ERR_INVALID
+    assert "8388608" in _lib.last_error() and "scratch" in _lib.last_error()
+    small = sph[:4].copy()
+    assert call(small, first_frame=0) == _lib.ATX_ERR_INVALID
+    assert call(small, accum_kind=_lib.ATX_ACCUM_KAHAN) == _lib.ATX_ERR_INVALID
+    assert "comp" in _lib.last_error()
