@@ -10,11 +10,23 @@ BVH.  The plain one-lane-per-pixel kernel and the flat loop over all spheres
 (the reference's algorithm) are timed on the same frames and reported in
 `variants`.  All four produce bit-identical images.
 
+The default run also carries, in the same JSON line:
+  batch       the 64-scene mixed workload: batched persistent renderer
+              (autotuned, hipGraph) vs the plain kernel rendering the scenes
+              one by one with the same sphere acceleration; `speedup`
+  serve       Poisson request streams through the serving loop at 50, 100,
+              200 and 400 requests/s (samples/s, p50/p99 latency, HBM
+              high-water mark per rate)
+  cpu_baseline  the oracle on the host cores over frames of the same scene
+
 Other workloads (one JSON line each):
-  --workload batch_mix   64 mixed scenes: batched persistent renderer vs the
-                         plain kernel rendering the scenes one by one
-  --workload serve       Poisson request stream through the serving loop
-                         (samples/s, p50/p99 latency, HBM high-water mark)
+  --workload batch_mix   the batch leg alone (with its own knobs)
+  --workload serve       one Poisson rate (--rate, --requests)
+  --workload precision   naive vs Kahan FP32 accumulation against a float64
+                         host sum at 1024x1024 and 4096 spp
+  --workload copy        a device-to-device copy of a known size: the
+                         reference reading for rocprofv3's FETCH_SIZE and
+                         WRITE_SIZE counters
 
 The roofline entry is the VALU bound of the dominant kernel: 17 algorithmic
 FLOP per ray-sphere test and 12 per BVH node box test (6 FMA; FMA = 2), both
@@ -45,8 +57,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256, help="timed frames")
     ap.add_argument("--warmup", type=int, default=8, help="untimed frames")
-    ap.add_argument("--workload", choices=["plain_1080p", "batch_mix", "serve", "precision"], default="plain_1080p")
-    ap.add_argument("--precision-size", type=int, default=128, help="precision: image side in pixels")
+    ap.add_argument("--workload", choices=["plain_1080p", "batch_mix", "serve", "precision", "copy"],
+                    default="plain_1080p")
+    ap.add_argument("--precision-size", type=int, default=1024, help="precision: image side in pixels")
     ap.add_argument("--precision-spp", type=int, default=4096, help="precision: samples per pixel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time budget of the CPU baseline")
@@ -65,30 +78,62 @@ def parse():
                     help="serve: frames per request per launch (0: the server's default)")
     ap.add_argument("--tick-mpaths", type=int, default=0,
                     help="serve: million paths per launch before admitting more (0: the server's default)")
-    ap.add_argument("--no-serve", action="store_true", help="plain_1080p: skip the short serving run")
+    ap.add_argument("--no-serve", action="store_true", help="plain_1080p: skip the serving runs")
+    ap.add_argument("--no-batch", action="store_true", help="plain_1080p: skip the batch leg")
+    ap.add_argument("--serve-rates", default="50,100,200,400", help="plain_1080p: Poisson rates of the serve leg")
+    ap.add_argument("--serve-requests", type=int, default=128, help="plain_1080p: requests per serve rate")
+    ap.add_argument("--copy-bytes", type=int, default=268435456, help="copy: bytes per copy")
     return ap.parse_args()
 
 
-def traffic_from_profile(name, kernel):
-    """HBM bytes per launch of `kernel` from a committed rocprof summary
-    (its FETCH_SIZE + WRITE_SIZE lines, calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path):
-        return None
-    mb = [float(m) for line in open(path) if re.search(rf"::{kernel}[(<]", line)
-          for m in re.findall(r"= ([0-9.]+) MB \(calibrated\)", line)]
-    return sum(mb) * 1e6 if len(mb) >= 2 else None
+def traffic_from_profile(name, kernel, frames):
+    """HBM bytes per launch of `kernel` for a launch of `frames` frames, from
+    a committed rocprof summary (its FETCH_SIZE + WRITE_SIZE lines,
+    calibrated), or None.  Summaries are named ..._f<frames>.txt after the
+    launch length they profiled; the one of the same length is read as is,
+    otherwise the closest one is scaled per frame (the path kernel's traffic
+    is the samples it writes and the first-bounce records it reads, both
+    proportional to the frames of the launch).  Returns (bytes, basis)."""
+    best = None
+    for fn in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+        m = re.fullmatch(re.escape(name) + r"_f(\d+)\.txt", fn)
+        if m:
+            f = int(m.group(1))
+            if best is None or abs(f - frames) < abs(best[0] - frames):
+                best = (f, fn)
+    if best is None:
+        return None, None
+    mb = [float(m) for line in open(os.path.join(ROOT, "profiles", best[1]))
+          if re.search(rf"::{kernel}[(<]", line) for m in re.findall(r"= ([0-9.]+) MB \(calibrated\)", line)]
+    if len(mb) < 2:
+        return None, None
+    pf = best[0]
+    if pf == frames:
+        return sum(mb) * 1e6, f"profiles/{best[1]} (a {pf}-frame launch, as timed here)"
+    return sum(mb) * 1e6 * frames / pf, f"profiles/{best[1]} (a {pf}-frame launch) scaled to {frames} frames"
+
+
+def host_cores():
+    """Cores this process may run on, and the host's count.  A GPU box grants
+    each GPU a share of 16 host cores, so the baseline uses at most 16."""
+    visible = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = visible
+    return max(1, min(16, allowed)), allowed, visible
 
 
 def cpu_baseline(sc, target_s):
     """The oracle on the host: whole frames of the same scene, row-parallel over
-    min(16, cores) threads, repeated until about `target_s` seconds elapse."""
+    every core the process may use (at most 16, a GPU's share of the box),
+    repeated until about `target_s` seconds elapse."""
     import numpy as np
     import oracle as O
     cam = O.camera_from(sc.camera)
     osc = O.Scene(sc.spheres, sc.materials, sc.lights)
     st = O.settings(sc.settings.max_bounces, sc.settings.sky_light, sc.settings.accumulation)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, allowed, visible = host_cores()
     n = cam.width * cam.height
     acc = np.zeros((n, 4), np.float32)
     frames, rays = 0, 0
@@ -108,13 +153,20 @@ def cpu_baseline(sc, target_s):
     return {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full frame(s) of the same 1080p workload ({samples} samples, {rays} rays) "
                       f"in {dt:.2f} s on {threads} threads; one more frame on 1 thread in {dt1:.2f} s",
-            "single_thread_value": n / dt1}
+            "single_thread_value": n / dt1, "cores_allowed": allowed, "nproc": visible,
+            "cores_note": "threads = the cores this process may use, at most 16 (one GPU's share of the box)"}
 
 
 def time_single_scene(sc, kernel, accel, steps, warmup):
+    """Times `steps` frames of one render call.  The warm-up frames use a
+    camera moved by 1e-3, so that the renderer's buffers exist but its
+    per-camera work (primary directions, first-bounce cache) is redone
+    inside the timed region, as a fresh camera would need it."""
+    import dataclasses
     from ataraxia_amd import Renderer
     from ataraxia_amd._lib import lib
     cam = sc.camera
+    warm_cam = dataclasses.replace(cam, position=(cam.position[0] + 1e-3,) + tuple(cam.position[1:]))
     r = Renderer()
     r.set_kernel(kernel)
     r.set_accel(accel)
@@ -122,7 +174,7 @@ def time_single_scene(sc, kernel, accel, steps, warmup):
     r.set_settings(sc.settings)
     r.set_scene(sc)
     if warmup:
-        r.render_frames(cam, warmup)
+        r.render_frames(warm_cam, warmup)
     r.reset_frame_index()
     lib.atx_device_synchronize()
     t0 = time.perf_counter()
@@ -138,12 +190,13 @@ def algo_flops(st):
     return FLOP_PER_TEST * st["sphere_tests"] + FLOP_PER_BOX * st["box_tests"]
 
 
-def roofline(st, traffic):
+def roofline(st, traffic, basis=None):
     launches = max(1, st["launches"])
     kernel_ms = st["path_ms"] / launches
     achieved = algo_flops(st) / launches / (kernel_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": kernel_ms,
+            "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": traffic, "traffic_basis": basis,
+            "kernel_ms": kernel_ms,
             "launches": launches, "sphere_tests_per_launch": st["sphere_tests"] / launches,
             "box_tests_per_launch": st["box_tests"] / launches, "rays_per_launch": st["rays"] / launches,
             "reused_rays_per_launch": st["reused_rays"] / launches,
@@ -164,10 +217,12 @@ def bench_plain_1080p(args):
     for k in kernels:
         for a in accels:
             wall, st = time_single_scene(sc, kid[k], aid[a], args.steps, args.warmup)
-            prof = f"plain_1080p_k_render_{k}_{a}.txt"
+            frames_per_launch = args.steps / max(1, st["launches"])
+            traffic, basis = traffic_from_profile(f"plain_1080p_k_render_{k}_{a}", f"k_render_{k}",
+                                                  int(round(frames_per_launch)))
             variants[f"{k}/{a}"] = {"value": samples / wall, "ms_per_step": wall * 1e3 / args.steps,
                                     "gpu_ms_per_step": st["gpu_ms"] / args.steps,
-                                    "roofline": roofline(st, traffic_from_profile(prof, f"k_render_{k}"))}
+                                    "roofline": roofline(st, traffic, basis)}
     main_variant = next(iter(variants))   # wavefront/bvh unless restricted
     v = variants[main_variant]
     out = {
@@ -190,8 +245,12 @@ def bench_plain_1080p(args):
         "roofline": v["roofline"],
         "variants": variants,
     }
+    if not args.no_batch:
+        bm = bench_batch_mix(args)
+        out["batch"] = {k: bm[k] for k in ("config", "batched", "plain", "speedup")}
     if not args.no_serve:
-        out["serve"] = bench_serve(args)
+        out["serve"] = [bench_serve(args, rate=float(r), requests=args.serve_requests)
+                        for r in args.serve_rates.split(",")]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     return out
@@ -348,6 +407,39 @@ def bench_precision(args):
             "kahan": ek, "naive": en, "kahan_below_1e-6": ek["max"] < 1e-6}
 
 
+def bench_copy(args):
+    """Device-to-device copies of --copy-bytes bytes through the HIP runtime:
+    with rocprofv3 --pmc FETCH_SIZE (or WRITE_SIZE) each copy's reading is
+    the counter's value for that many bytes read (or written)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = args.copy_bytes
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), n) != 0 or hip.hipMalloc(C.byref(dst), n) != 0:
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, n) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        copies = max(1, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(copies):
+            if hip.hipMemcpy(dst, src, n, 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return {"metric": "device-to-device copy (counter calibration)", "value": 2 * n * copies / wall / 1e9,
+            "unit": "GB/s (read + write)", "n_gpus": 1, "steps": copies, "warmup": 0,
+            "ms_per_step": wall * 1e3 / copies, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic", "config": {"workload": f"copy: {n} bytes per copy"}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -362,6 +454,8 @@ def main():
         out = bench_batch_mix(args)
     elif args.workload == "precision":
         out = bench_precision(args)
+    elif args.workload == "copy":
+        out = bench_copy(args)
     else:
         sv = bench_serve(args)
         out = {"metric": "path samples/s + p50/p99 request latency in serve loop", "value": sv["samples_per_s"],

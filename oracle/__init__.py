@@ -65,6 +65,8 @@ _PROTOS = {
     "ora_rgba8": (_U32, [_P]),
     "ora_render_frame_debug": (None, [C.POINTER(OraCamera), C.POINTER(OraScene), C.POINTER(OraSettings), _U32,
                                       _P, _P, _P, _U32, _P, _U32, _U32]),
+    "ora_accumulate_pixels": (C.c_int, [C.POINTER(OraCamera), C.POINTER(OraScene), C.POINTER(OraSettings), _U32,
+                                        _U32, _P, _U64, C.c_int, _P, _P, C.c_int]),
     "ora_node_local_transform": (None, [_P, _P, _P, _P]),
     "ora_mat4_mul": (None, [_P, _P, _P]),
     "ora_flatten_sphere": (None, [_P, _P, _P]),
@@ -213,6 +215,21 @@ def render_frame_debug(cam: OraCamera, scene: Scene, sett: OraSettings, frame: i
     lib.ora_render_frame_debug(C.byref(cam), C.byref(scene.c), C.byref(sett), frame, _p(rad), _p(hit), _p(sh),
                                max_bounces, _p(rays), r0, r1)
     return rad, hit, sh, rays
+
+
+def accumulate_pixels(cam: OraCamera, scene: Scene, sett: OraSettings, first_frame: int, frames: int, pixels,
+                      kahan: bool = False, threads: int = 8):
+    """Accumulation (4 floats per listed pixel) and RGBA8 after `frames`
+    frames from first_frame, for pixel indices x + y*width."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint32)
+    acc = np.zeros((len(px), 4), np.float32)
+    img = np.zeros(len(px), np.uint32)
+    if len(px):
+        rc = lib.ora_accumulate_pixels(C.byref(cam), C.byref(scene.c), C.byref(sett), first_frame, frames,
+                                       _p(px), len(px), 1 if kahan else 0, _p(acc), _p(img), threads)
+        if rc != 0:
+            raise RuntimeError("oracle thread creation failed")
+    return acc, img
 
 
 def pcg_hash(seed: int) -> int:

@@ -713,6 +713,76 @@ int ora_render_frames_mt(const ora_camera* cam, const ora_scene* sc, const ora_s
     return 0;
 }
 
+typedef struct {
+    const ora_camera* cam; const ora_scene* sc; const ora_settings* settings;
+    const float* ip; const float* iv;
+    uint32_t first_frame, frames; const uint32_t* pixels; uint64_t n; int kahan;
+    float* accum; uint32_t* image; volatile uint64_t* next;
+} px_job;
+
+static void* px_worker(void* arg)
+{
+    px_job* j = (px_job*)arg;
+    uint32_t W = j->cam->width;
+    for (;;) {
+        uint64_t i = __sync_fetch_and_add(j->next, 1u);
+        if (i >= j->n)
+            break;
+        uint32_t x = j->pixels[i] % W, y = j->pixels[i] / W;
+        float d[3], acc[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
+        ora_ray_direction(j->ip, j->iv, x, y, W, j->cam->height, d);
+        /* Renderer.cu:245-248: the index advances with accumulation, else stays 1 */
+        uint32_t fi = j->first_frame;
+        for (uint32_t f = 0; f < j->frames; ++f) {
+            float c[4];
+            ora_per_pixel(x, y, W, d, j->cam->position, j->sc, fi, j->settings, c, 0);
+            if (fi == 1u)
+                for (int k = 0; k < 4; ++k) acc[k] = cp[k] = 0.0f;
+            for (int k = 0; k < 4; ++k) {
+                if (j->kahan) {
+                    float yk = c[k] - cp[k];
+                    float t = acc[k] + yk;
+                    cp[k] = (t - acc[k]) - yk;
+                    acc[k] = t;
+                } else {
+                    acc[k] = acc[k] + c[k];
+                }
+            }
+            if (f + 1 < j->frames)
+                fi = j->settings->accumulation ? fi + 1u : 1u;
+        }
+        memcpy(j->accum + 4 * i, acc, sizeof(acc));
+        if (j->image) {
+            float fin[4];
+            for (int k = 0; k < 4; ++k)
+                fin[k] = gclamp(acc[k] / (float)fi, 0.0f, 1.0f);
+            j->image[i] = ora_rgba8(fin);
+        }
+    }
+    return 0;
+}
+
+int ora_accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_settings* settings,
+                          uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                          int kahan, float* accum, uint32_t* image, int threads)
+{
+    float ip[16], iv[16];
+    ora_camera_matrices(cam, ip, iv);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    volatile uint64_t next = 0;
+    pthread_t tid[256];
+    px_job job = {cam, sc, settings, ip, iv, first_frame, frames, pixels, n, kahan, accum, image, &next};
+    for (int t = 0; t < threads; ++t)
+        if (pthread_create(&tid[t], 0, px_worker, &job) != 0) {
+            for (int u = 0; u < t; ++u) pthread_join(tid[u], 0);
+            return -1;
+        }
+    for (int t = 0; t < threads; ++t)
+        pthread_join(tid[t], 0);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* a11: scene-graph flattening (Renderer.cu:67-96, SceneNode.cpp:42-59)       */
 /* ------------------------------------------------------------------------- */

@@ -123,6 +123,16 @@ int  ora_render_frames_mt(const ora_camera* cam, const ora_scene* scene, const o
 
 uint32_t ora_rgba8(const float rgba[4]);
 
+/* Accumulation of `frames` consecutive frames (first_frame, first_frame+1,
+ * ...; the reference's frame order) for a list of pixels (x + y*width),
+ * as Renderer::Render leaves it: accum[4*i..] for pixels[i], the naive sum
+ * of each frame's sample, or the Kahan-compensated one when kahan != 0.
+ * image (optional) gets RGBA8 of accum / last frame index.  Runs on
+ * `threads` POSIX threads over the pixel list.                            */
+int  ora_accumulate_pixels(const ora_camera* cam, const ora_scene* scene, const ora_settings* settings,
+                           uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                           int kahan, float* accum, uint32_t* image, int threads);
+
 /* Per-pixel debug capture of one frame over rows [row0,row1): radiance (4
  * floats per pixel), continuation and shadow hit ids per bounce (pixel-major,
  * max_bounces per pixel, -2 = not traced), rays per pixel.  Any output may be

@@ -323,3 +323,29 @@ def test_accumulation_mean_and_kahan():
         assert np.array_equal(img, exp_img)
     # Kahan's sum is at least as close to the float64 sum as the naive one
     assert np.abs(kacc - s64).sum() <= np.abs(acc - s64).sum()
+
+
+def test_accumulate_pixels_equals_frame_by_frame():
+    """The per-pixel accumulation the workload tests compare against equals
+    the frame-by-frame oracle (naive and Kahan, accumulation on and off)."""
+    from ataraxia_amd import plain_cpu_scene
+    from ataraxia_amd.scene import scene_from_camera_overrides
+    sc = scene_from_camera_overrides(plain_cpu_scene(), 20, 12)
+    cam = O.camera_from(sc.camera)
+    osc = O.Scene(sc.spheres, sc.materials, sc.lights)
+    n = 20 * 12
+    pix = np.arange(n)[::3]
+    for kahan in (False, True):
+        st = O.settings(5, False, True)
+        acc = np.zeros((n, 4), np.float32)
+        comp = np.zeros((n, 4), np.float32) if kahan else None
+        for f in range(1, 6):
+            acc, comp, img, _, _ = O.render_frame(cam, osc, st, f, accum=acc, comp=comp)
+        got, gimg = O.accumulate_pixels(cam, osc, st, 1, 5, pix, kahan=kahan, threads=3)
+        assert np.array_equal(got, acc[pix]) and np.array_equal(gimg, img[pix])
+        # accumulation off: the index stays 1 and every frame restarts the sum
+        st = O.settings(5, False, False)
+        one, _, img1, _, _ = O.render_frame(cam, osc, st, 1, accum=np.zeros((n, 4), np.float32),
+                                            comp=np.zeros((n, 4), np.float32) if kahan else None)
+        got, gimg = O.accumulate_pixels(cam, osc, st, 1, 5, pix, kahan=kahan, threads=3)
+        assert np.array_equal(got, one[pix]) and np.array_equal(gimg, img1[pix])
