@@ -11,6 +11,7 @@
 // keys in lexicographic order, 4-space indent, one array element per line,
 // numbers as the shortest decimal that round-trips the float's double value.
 #include <errno.h>
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -424,6 +425,31 @@ static void flatten(const SceneGraph& g, uint32_t node, const float parent[16], 
 }
 
 // ---- (de)serialisation (Utils.cpp:3-173) ----
+// Numbers are read as doubles and narrowed to the reference's float and int
+// fields.  A value outside the target's range is rejected: converting it
+// would be undefined behaviour, and a float that overflowed to infinity
+// would be written back as null (nlohmann's dump of a non-finite number),
+// which no reader accepts.
+static bool to_float(double d, float* out, std::string* err, const char* what)
+{
+    if (!(std::fabs(d) <= (double)FLT_MAX)) {
+        *err = std::string("number out of the float range in ") + what;
+        return false;
+    }
+    *out = (float)d;
+    return true;
+}
+
+static bool to_int32(double d, int32_t* out, std::string* err, const char* what)
+{
+    if (!(d > -2147483649.0 && d < 2147483648.0)) {
+        *err = std::string("number out of the int range in ") + what;
+        return false;
+    }
+    *out = (int32_t)d;   // truncates, as a float-to-int conversion does
+    return true;
+}
+
 static bool read_vec(const Json* j, int n, float* out, std::string* err, const char* what)
 {
     if (!j || j->type != Json::Array || (int)j->arr.size() < n) {
@@ -435,7 +461,7 @@ static bool read_vec(const Json* j, int n, float* out, std::string* err, const c
             *err = std::string("non-numeric entry in ") + what;
             return false;
         }
-        out[i] = (float)j->arr[(size_t)i].num;
+        if (!to_float(j->arr[(size_t)i].num, &out[i], err, what)) return false;
     }
     return true;
 }
@@ -471,10 +497,10 @@ static bool read_node(SceneGraph& g, const Json& j, uint32_t idx, std::string* e
             double r = 0, id = 0;
             if (!read_vec(s.get("center"), 3, o.center, err, "sphere.center") ||
                 !read_num(s.get("radius"), &r, err, "sphere.radius") ||
-                !read_num(s.get("materialIndex"), &id, err, "sphere.materialIndex"))
+                !read_num(s.get("materialIndex"), &id, err, "sphere.materialIndex") ||
+                !to_float(r, &o.radius, err, "sphere.radius") ||
+                !to_int32(id, &o.material, err, "sphere.materialIndex"))
                 return false;
-            o.radius = (float)r;
-            o.material = (int32_t)id;
             g.nodes[idx].spheres.push_back(o);
         }
     }
@@ -499,14 +525,14 @@ static bool from_json(const Json& j, SceneGraph* g, std::string* err)
     double fov = 45.0;
     if (!cam || !read_vec(cam->get("position"), 3, g->cam_pos, err, "camera.position") ||
         !read_vec(cam->get("direction"), 3, g->cam_dir, err, "camera.direction") ||
-        !read_num(cam->get("fov"), &fov, err, "camera.fov")) {
+        !read_num(cam->get("fov"), &fov, err, "camera.fov") || !to_float(fov, &g->fov, err, "camera.fov")) {
         if (err->empty()) *err = "missing camera";
         return false;
     }
-    g->fov = (float)fov;
     if (const Json* sg = j.get("sceneGraph"))
         if (!read_node(*g, *sg, 0, err, 0)) return false;
     if (const Json* ms = j.get("materials")) {
+        if (ms->type != Json::Array) { *err = "materials is not an array"; return false; }
         for (const Json& m : ms->arr) {
             atx_material o{};
             o.albedo[0] = o.albedo[1] = o.albedo[2] = 1.0f;
@@ -517,24 +543,25 @@ static bool from_json(const Json& j, SceneGraph* g, std::string* err)
                 !read_num(m.get("metallic"), &metal, err, "material.metallic") ||
                 !read_vec(m.get("F0"), 3, o.F0, err, "material.F0") ||
                 !read_num(m.get("emissionIntensity"), &ei, err, "material.emissionIntensity") ||
-                !read_vec(m.get("emissionColor"), 3, o.emission_color, err, "material.emissionColor"))
+                !read_vec(m.get("emissionColor"), 3, o.emission_color, err, "material.emissionColor") ||
+                !to_float(rough, &o.roughness, err, "material.roughness") ||
+                !to_float(metal, &o.metallic, err, "material.metallic") ||
+                !to_float(ei, &o.emission_intensity, err, "material.emissionIntensity"))
                 return false;
-            o.roughness = (float)rough;
-            o.metallic = (float)metal;
-            o.emission_intensity = (float)ei;
             o.id = 0;
             g->materials.push_back(o);
         }
     }
     if (const Json* ls = j.get("lights")) {
+        if (ls->type != Json::Array) { *err = "lights is not an array"; return false; }
         for (const Json& l : ls->arr) {
             atx_light o{};
             double in = 0;
             if (!read_vec(l.get("position"), 3, o.position, err, "light.position") ||
                 !read_num(l.get("intensity"), &in, err, "light.intensity") ||
-                !read_vec(l.get("color"), 3, o.color, err, "light.color"))
+                !read_vec(l.get("color"), 3, o.color, err, "light.color") ||
+                !to_float(in, &o.intensity, err, "light.intensity"))
                 return false;
-            o.intensity = (float)in;
             g->lights.push_back(o);
         }
     }
@@ -546,7 +573,7 @@ static bool from_json(const Json& j, SceneGraph* g, std::string* err)
         if (err->empty()) *err = "missing settings";
         return false;
     }
-    g->settings.max_bounces = (int32_t)mb;
+    if (!to_int32(mb, &g->settings.max_bounces, err, "settings.maxBounces")) return false;
     g->settings.sky_light = sky != 0.0;
     g->settings.accumulation = acc != 0.0;
     return true;

@@ -99,6 +99,7 @@ struct Server {
     bool have_prev = false;         // the previous launch ran without an idle period since
     DevBuf table, queue_ctr, counters;
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
+    size_t samples_excess = 0;      // its capacity beyond the active requests' shares, charged to hbm_in_use
     double rendered = 0.0, busy_s = 0.0;   // samples rendered and GPU seconds so far (worker)
     uint32_t grid = 0, grid_flat = 0;   // persistent grids: ticks with trees, flat-loop-only ticks
     int device = 0;
@@ -362,8 +363,23 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
         table.push_back(w);
     }
     WfPass wp{};
-    bool ok = s->samples.reserve(std::max<size_t>(1, launch_samples(table.data(), table.size())) *
-                                 sizeof(float4)) == hipSuccess;
+    // The buffer grows to the largest tick since the server last idled (a
+    // hipFree and hipMalloc per shrink would stall the device between
+    // ticks).  Its capacity beyond the shares the active requests are
+    // charged (16 B per pixel and frame of a tick) is charged to the budget
+    // as well, so hbm_in_use counts every byte the server holds.
+    const size_t need = std::max<size_t>(1, launch_samples(table.data(), table.size())) * sizeof(float4);
+    bool ok = s->samples.reserve(need) == hipSuccess;
+    {
+        size_t shares = 0;
+        for (auto& r : in_tick)
+            shares += (size_t)r->cam.width * r->cam.height * std::min<size_t>(F, r->spp) * sizeof(float4);
+        const size_t excess = s->samples.bytes() > shares ? s->samples.bytes() - shares : 0;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.hbm_in_use = s->stats.hbm_in_use - s->samples_excess + excess;
+        s->stats.hbm_high_water = std::max(s->stats.hbm_high_water, s->stats.hbm_in_use);
+        s->samples_excess = excess;
+    }
     const bool flat = wf_flat_only(table.data(), table.size());
     const WfLaunch shape{flat ? s->grid_flat : s->grid, flat};
     const uint32_t P =
@@ -494,6 +510,12 @@ static void worker_main(Server* s)
         }
         if (idle) {
             release_returned(s);   // nothing runs: free retired requests' memory, then admit
+            s->samples.release();  // and the tick's sample buffer, sized for ticks that have ended
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->stats.hbm_in_use -= s->samples_excess;
+                s->samples_excess = 0;
+            }
             admit_queued(s);
             s->have_prev = false;  // the device idled for want of work: no gap to count
         }

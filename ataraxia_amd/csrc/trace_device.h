@@ -136,7 +136,6 @@ __device__ __forceinline__ Hit trace_filtered(const Src& src, uint32_t n, f3 o, 
 {
     const float a = dot(d, d);
     Filter f = filter_init();
-#pragma unroll 2
     for (uint32_t i = 0; i < n; ++i)
         filter_sphere(src(i), (int32_t)i, o, d, a, f);
     const bool amb = f.amb;

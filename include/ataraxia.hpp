@@ -16,6 +16,7 @@
 #include <array>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -317,21 +318,36 @@ public:
         m_width = width;
         m_height = height;
     }
-    // Renderer.cu:173-249: the scene is re-uploaded when it changes or at frame 1.
+    // Renderer.cu:173-249: the reference re-uploads the scene when the Scene
+    // object changes or the frame index is 1 (every frame with accumulation
+    // off).  Here the scene is flattened at those points as in the reference,
+    // and uploaded only when the flattened spheres, materials or lights differ
+    // from what the device holds: the device then holds exactly what the
+    // reference's upload would give, without rebuilding the sphere tree and
+    // the first-bounce cache every frame.
     void Render(Camera& camera, const Scene& scene)
     {
         uint32_t fi = 0;
         check(atx_renderer_get_frame_index(m_r, &fi));
         if (m_scene != &scene || fi == 1) {
             m_scene = &scene;
-            const std::vector<Sphere> sph = detail::flatten(scene);
-            check(atx_renderer_set_scene(m_r, sph.data(), sph.size(), scene.materials.data(), scene.materials.size(),
-                                         scene.lights.data(), scene.lights.size()));
+            std::vector<Sphere> sph = detail::flatten(scene);
+            if (!m_uploaded || !same(sph, m_sph) || !same(scene.materials, m_mat) || !same(scene.lights, m_lig)) {
+                check(atx_renderer_set_scene(m_r, sph.data(), sph.size(), scene.materials.data(),
+                                             scene.materials.size(), scene.lights.data(), scene.lights.size()));
+                m_sph = std::move(sph);
+                m_mat = scene.materials;
+                m_lig = scene.lights;
+                m_uploaded = true;
+                ++m_uploads;
+            }
         }
         camera.Resize(m_width, m_height);
         const atx_camera c = camera.desc();
         check(atx_renderer_render(m_r, &c));
     }
+    // Scene uploads so far (each one rebuilds the device scene and its tree).
+    uint64_t sceneUploads() const { return m_uploads; }
     // The RGBA8 pixels the reference uploads to its Vulkan Image.
     std::vector<uint32_t> getImage() const
     {
@@ -354,8 +370,18 @@ public:
     atx_renderer* handle() { return m_r; }
 
 private:
+    template <class T>
+    static bool same(const std::vector<T>& a, const std::vector<T>& b)
+    {
+        return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(T)) == 0);
+    }
     atx_renderer* m_r = nullptr;
     const Scene* m_scene = nullptr;
+    std::vector<Sphere> m_sph;
+    std::vector<Material> m_mat;
+    std::vector<Light> m_lig;
+    bool m_uploaded = false;
+    uint64_t m_uploads = 0;
     uint32_t m_width = 0, m_height = 0;
 };
 

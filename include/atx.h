@@ -11,9 +11,9 @@
  *   atx_renderer_on_resize               Renderer::onResize                  Engine/src/Renderer.cu:98-146
  *   atx_renderer_set_scene               Renderer::allocateDeviceMemory      Engine/src/Renderer.cu:25-57
  *   atx_renderer_render                  Renderer::Render                    Engine/src/Renderer.cu:173-249
- *   atx_renderer_get_image               Renderer::getImage + Image pixels   Engine/include/Renderer.h:23, Renderer.cu:240-242
- *   atx_renderer_get/set_settings        Renderer::getSettings/setSettings   Engine/include/Renderer.h:25-26
- *   atx_renderer_reset_frame_index       Renderer::resetFrameIndex           Engine/include/Renderer.h:28
+ *   atx_renderer_get_image               Renderer::getImage + Image pixels   Engine/include/Renderer.h:26, Renderer.cu:240-242
+ *   atx_renderer_get/set_settings        Renderer::getSettings/setSettings   Engine/include/Renderer.h:28-29
+ *   atx_renderer_reset_frame_index       Renderer::resetFrameIndex           Engine/include/Renderer.h:31
  *   atx_renderer_set_accel               Renderer::traceRay sphere loop      Engine/src/Renderer.cu:251-285
  *   atx_camera_ray_directions            Camera::UpdateRayDirection          Engine/src/Camera.cpp:161-195
  *   atx_scene_* (scene.json)             Utils::importScene / exportScene    Engine/src/Utils.cpp:53-62,175-187
@@ -74,6 +74,25 @@ typedef struct atx_camera {
     uint32_t width;
     uint32_t height;
 } atx_camera;
+
+/* The layouts above are the ABI: checked at compile time in C and C++. */
+#ifdef __cplusplus
+#define ATX_STATIC_ASSERT(cond, msg) static_assert(cond, msg)
+#else
+#define ATX_STATIC_ASSERT(cond, msg) _Static_assert(cond, msg)
+#endif
+ATX_STATIC_ASSERT(sizeof(atx_sphere) == 20 && offsetof(atx_sphere, radius) == 12 &&
+                  offsetof(atx_sphere, material) == 16, "atx_sphere must be the packed 20-byte Sphere");
+ATX_STATIC_ASSERT(sizeof(atx_material) == 52 && offsetof(atx_material, roughness) == 12 &&
+                  offsetof(atx_material, metallic) == 16 && offsetof(atx_material, F0) == 20 &&
+                  offsetof(atx_material, emission_color) == 32 && offsetof(atx_material, emission_intensity) == 44 &&
+                  offsetof(atx_material, id) == 48, "atx_material must be the packed 52-byte Material");
+ATX_STATIC_ASSERT(sizeof(atx_light) == 28 && offsetof(atx_light, color) == 12 && offsetof(atx_light, intensity) == 24,
+                  "atx_light must be the packed 28-byte Light");
+ATX_STATIC_ASSERT(sizeof(atx_settings) == 8 && offsetof(atx_settings, sky_light) == 1 &&
+                  offsetof(atx_settings, max_bounces) == 4, "atx_settings must be 8 bytes");
+ATX_STATIC_ASSERT(sizeof(atx_camera) == 44 && offsetof(atx_camera, fov_deg) == 24 &&
+                  offsetof(atx_camera, width) == 36, "atx_camera layout");
 
 /* Accumulation buffer arithmetic.  NAIVE is the reference's `accum += c`
  * (Renderer.cu:165); KAHAN keeps a compensation term per channel.           */

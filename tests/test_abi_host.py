@@ -159,3 +159,33 @@ def test_noalloc_entry_rejects_before_touching_the_device():
     assert call(small, first_frame=0) == _lib.ATX_ERR_INVALID
     assert call(small, accum_kind=_lib.ATX_ACCUM_KAHAN) == _lib.ATX_ERR_INVALID
     assert "comp" in _lib.last_error()
+
+
+def test_ctypes_layouts_equal_the_c_compilers(tmp_path):
+    """Every struct of atx.h as the C compiler lays it out (sizeof and each
+    field's offsetof, printed by a small program built here) equals the
+    ctypes mirror Python reads it through."""
+    pairs = {"atx_sphere": _lib.Sphere, "atx_material": _lib.Material, "atx_light": _lib.Light,
+             "atx_settings": _lib.Settings, "atx_camera": _lib.CameraDesc, "atx_scene_desc": _lib.SceneDesc,
+             "atx_batch_tuning": _lib.BatchTuning, "atx_server_config": _lib.ServerConfig,
+             "atx_server_stats": _lib.ServerStats, "atx_request_info": _lib.RequestInfo,
+             "atx_render_stats": _lib.RenderStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "atx.h"', "int main(void) {"]
+    for cname, cls in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layouts.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layouts"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o",
+                    str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in pairs.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

@@ -242,3 +242,26 @@ def test_flatten_parent_times_child():
     _, outs = _oracle_node_flatten(G, (1, 0, 0), [0, 0, 0, 1], (3, 3, 3), [((0, 0, 0), 1.0), ((1, 0, 0), 0.5)])
     for o, got in zip(outs, s):
         assert np.array_equal(o["center"], got["center"]) and o["radius"] == got["radius"]
+
+
+def test_numbers_outside_the_fields_range_are_rejected():
+    """A float field that would overflow (it would be written back as null,
+    which no reader accepts) or an int field outside int32 is a parse error,
+    not undefined behaviour; whatever parses writes back and re-parses."""
+    text = open(os.path.join(GOLDEN, "sample_scene.json")).read()
+    for old, new, field in (("0.03999999910593033", "1e39", "material.F0"),
+                            ('"radius": 100.0', '"radius": -1e300', "sphere.radius"),
+                            ('"materialIndex": 1', '"materialIndex": 3e9', "sphere.materialIndex"),
+                            ('"maxBounces": 25', '"maxBounces": 1e10', "settings.maxBounces")):
+        assert old in text, old
+        with pytest.raises(AtxError) as e:
+            SceneGraph.parse_json(text.replace(old, new, 1))
+        assert e.value.code == _lib.ATX_ERR_IO and field in str(e.value) and "range" in str(e.value)
+    g = SceneGraph.parse_json(text.replace('"radius": 100.0', '"radius": 3.4e38', 1))
+    assert SceneGraph.parse_json(g.to_json()).flatten()["radius"][1] == np.float32(3.4e38)
+    import json
+    doc = json.loads(text)
+    doc["materials"] = {"a": doc["materials"]}
+    with pytest.raises(AtxError) as e:
+        SceneGraph.parse_json(json.dumps(doc))
+    assert "materials is not an array" in str(e.value)
