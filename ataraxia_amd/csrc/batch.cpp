@@ -131,7 +131,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
             BatchEntry& e = *b->scenes[b->order[k]];
             const uint32_t spp = std::min(e.spp, max_frames);
             if (spp <= f0) break;
-            const size_t bytes = (size_t)e.cam.width * e.cam.height * std::min(F, spp - f0) * sizeof(float4);
+            const size_t bytes = (size_t)e.cam.width * e.cam.height * std::min(F, spp - f0) * kSampleBytes;
             if (group_bytes > 0 && group_bytes + bytes > kSampleBufferBytes) {
                 b->launch_scenes.push_back((uint32_t)(table.size() - b->launch_off.back()));
                 b->launch_off.push_back((uint32_t)table.size());
@@ -174,13 +174,13 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     size_t most = 0;
     for (size_t l = 0; l < b->launch_off.size(); ++l)
         most = std::max(most, launch_samples(table.data() + b->launch_off[l], b->launch_scenes[l]));
-    ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * sizeof(float4)));
+    ATX_HIP(b->d_samples.reserve(std::max<size_t>(1, most) * kSampleBytes));
     for (size_t l = 0; l < b->launch_off.size(); ++l) {
         const WfScene* lt = table.data() + b->launch_off[l];
         const uint32_t P = b->chunk_px ? b->chunk_px : auto_chunk_px(launch_jobs(lt, b->launch_scenes[l]), b->grid);
         WfPass lp{};
         if (!layout_launch(table.data() + b->launch_off[l], b->launch_scenes[l], P,
-                           b->d_samples.as<float4>(), &lp)) {
+                           b->d_samples.as<float3>(), &lp)) {
             set_error("batch: a launch of %u frames holds 2^31 or more paths; lower frames_per_launch", F);
             return ATX_ERR_INVALID;
         }

@@ -209,7 +209,7 @@ __device__ __forceinline__ bool next_bounce(CScene* S, int32_t mid, f3 nrm, f3 s
 // The path's sample (and the last frame's debug capture).
 __device__ __forceinline__ void store_sample(CScene* S, uint32_t frame, uint32_t px, f3 col)
 {
-    S->samples[(size_t)(frame - S->frame0) * (S->width * S->height) + px] = make_float4(col.x, col.y, col.z, 1.0f);
+    S->samples[(size_t)(frame - S->frame0) * (S->width * S->height) + px] = make_float3(col.x, col.y, col.z);
     if (S->dbg_radiance && frame == S->frame0 + S->frames - 1u)
         S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
 }
@@ -502,9 +502,11 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
             for (uint32_t f = 0; f < S.frames; ++f)
                 add(sky_c);
         } else {
-            const ATX_GLOBAL float4* sp = (const ATX_GLOBAL float4*)(uintptr_t)S.samples + px;
-            for (uint32_t f = 0; f < S.frames; ++f)
-                add(sp[(size_t)f * npx]);
+            const ATX_GLOBAL float3* sp = (const ATX_GLOBAL float3*)(uintptr_t)S.samples + px;
+            for (uint32_t f = 0; f < S.frames; ++f) {
+                const float3 c = sp[(size_t)f * npx];
+                add(make_float4(c.x, c.y, c.z, 1.0f));
+            }
         }
         S.comp[px] = cp;
     } else if (sky) {
@@ -515,26 +517,27 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
             acc.w = acc.w + sky_c.w;
         }
     } else {
-        // four frames' loads in flight ahead of their additions (in frame order)
-        const ATX_GLOBAL float4* sp = (const ATX_GLOBAL float4*)(uintptr_t)S.samples + px;
+        // four frames' loads in flight ahead of their additions (in frame
+        // order); every sample's alpha is 1
+        const ATX_GLOBAL float3* sp = (const ATX_GLOBAL float3*)(uintptr_t)S.samples + px;
         uint32_t f = 0;
         for (; f + 4u <= S.frames; f += 4u) {
-            float4 c[4];
+            float3 c[4];
             for (int k = 0; k < 4; ++k)
                 c[k] = sp[(size_t)(f + k) * npx];
             for (int k = 0; k < 4; ++k) {
                 acc.x = acc.x + c[k].x;
                 acc.y = acc.y + c[k].y;
                 acc.z = acc.z + c[k].z;
-                acc.w = acc.w + c[k].w;
+                acc.w = acc.w + 1.0f;
             }
         }
         for (; f < S.frames; ++f) {
-            const float4 c = sp[(size_t)f * npx];
+            const float3 c = sp[(size_t)f * npx];
             acc.x = acc.x + c.x;
             acc.y = acc.y + c.y;
             acc.z = acc.z + c.z;
-            acc.w = acc.w + c.w;
+            acc.w = acc.w + 1.0f;
         }
     }
     S.accum[px] = acc;

@@ -119,7 +119,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     WfLaunch shape{0, false};
     if (wf) {
         uint32_t fi = r->frame_index, left = frames;
-        const uint32_t fit = (uint32_t)std::max<size_t>(1, kSampleBufferBytes / (npx * sizeof(float4)));
+        const uint32_t fit = (uint32_t)std::max<size_t>(1, kSampleBufferBytes / (npx * kSampleBytes));
         const uint32_t per_launch = std::min(r->wf_frames, fit);
         while (left > 0) {
             const uint32_t k = r->settings.accumulation ? (left < per_launch ? left : per_launch) : 1u;
@@ -138,7 +138,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         // every launch runs after the previous one on the stream: one sample buffer serves all
         size_t most = 0;
         for (const WfScene& e : table) most = std::max(most, launch_samples(&e, 1));
-        ATX_HIP(r->wf_samples.reserve(most * sizeof(float4)));
+        ATX_HIP(r->wf_samples.reserve(most * kSampleBytes));
         shape.flat_only = wf_flat_only(table.data(), table.size());
         shape.workgroups = wavefront_full_grid(shape.flat_only);
         // The first-bounce cache pays from two frames per camera on: a call
@@ -171,7 +171,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(scene_jobs(e), shape.workgroups);
             launch_chunk.push_back(P);
             WfPass lp{};
-            if (!layout_launch(&e, 1, P, r->wf_samples.as<float4>(), &lp)) {
+            if (!layout_launch(&e, 1, P, r->wf_samples.as<float3>(), &lp)) {
                 set_error("render: %u x %u pixels x %u frames exceed one launch", e.width, e.height, e.frames);
                 return ATX_ERR_INVALID;
             }

@@ -123,7 +123,7 @@ static size_t request_bytes(const Request& r, bool kahan, uint32_t frames_per_ti
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return scene_device_bytes(r.spheres.size(), r.materials.size(), r.lights.size()) +
-           npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * 16 +
+           npx * (16 + (kahan ? 16 : 0) + 4 + 16) + npx * f * kSampleBytes +
            (r.spp >= 2 ? first_cache_bytes(npx, (uint32_t)r.lights.size()) : 0);
 }
 
@@ -136,7 +136,7 @@ static size_t held_bytes(const Request& r, uint32_t frames_per_tick)
     const size_t npx = (size_t)r.cam.width * r.cam.height;
     const size_t f = std::min<size_t>(frames_per_tick, r.spp);
     return r.geo.bytes() + r.accum.bytes() + r.comp.bytes() + r.image.bytes() + r.dirs.bytes() + r.first.bytes() +
-           npx * f * 16;
+           npx * f * kSampleBytes;
 }
 
 // A launch's paths must stay below 2^31 (the kernel's 32-bit job indices).
@@ -366,14 +366,14 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
     // The buffer grows to the largest tick since the server last idled (a
     // hipFree and hipMalloc per shrink would stall the device between
     // ticks).  Its capacity beyond the shares the active requests are
-    // charged (16 B per pixel and frame of a tick) is charged to the budget
+    // charged (kSampleBytes per pixel and frame of a tick) is charged to the budget
     // as well, so hbm_in_use counts every byte the server holds.
-    const size_t need = std::max<size_t>(1, launch_samples(table.data(), table.size())) * sizeof(float4);
+    const size_t need = std::max<size_t>(1, launch_samples(table.data(), table.size())) * kSampleBytes;
     bool ok = s->samples.reserve(need) == hipSuccess;
     {
         size_t shares = 0;
         for (auto& r : in_tick)
-            shares += (size_t)r->cam.width * r->cam.height * std::min<size_t>(F, r->spp) * sizeof(float4);
+            shares += (size_t)r->cam.width * r->cam.height * std::min<size_t>(F, r->spp) * kSampleBytes;
         const size_t excess = s->samples.bytes() > shares ? s->samples.bytes() - shares : 0;
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.hbm_in_use = s->stats.hbm_in_use - s->samples_excess + excess;
@@ -384,7 +384,7 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
     const WfLaunch shape{flat ? s->grid_flat : s->grid, flat};
     const uint32_t P =
         s->cfg.chunk_px ? s->cfg.chunk_px : auto_chunk_px(launch_jobs(table.data(), table.size()), shape.workgroups);
-    if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float4>(), &wp);
+    if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float3>(), &wp);
     ok = ok && s->table.reserve(table.size() * sizeof(WfScene)) == hipSuccess &&
          hipMemcpyAsync(s->table.get(), table.data(), table.size() * sizeof(WfScene), hipMemcpyHostToDevice,
                         s->stream) == hipSuccess &&

@@ -55,7 +55,8 @@ struct WfScene {
     float4* dbg_radiance;  // optional per-pixel capture of the last frame (nullptr = off)
     int32_t* dbg_hit;
     int32_t* dbg_shadow;
-    float4* samples;       // this launch's samples: [frames][width * height], (r, g, b, 1)
+    float3* samples;       // this launch's samples: [frames][width * height], (r, g, b);
+                           // k_accumulate adds the alpha of 1 itself
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
     const float4* first_light; // first-bounce light terms, [light][pixel] (nullptr: none)
@@ -80,6 +81,9 @@ struct WfPass {
 };
 
 constexpr uint32_t kMaxChunkPx = 1024;
+
+// Bytes of one path sample in the launch's sample buffer (r, g, b).
+constexpr size_t kSampleBytes = 12;
 
 // Job order.  A scene's jobs come in groups of kJobGroupPx neighbouring
 // pixels (the last group may hold fewer) times the launch's frames, frame

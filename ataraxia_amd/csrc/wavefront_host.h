@@ -42,7 +42,7 @@ inline void fill_wf_scene(const DevScene& g, const atx_settings& st, const atx_c
 }
 
 
-// Sample-buffer floats4 one launch of these table entries needs.
+// Samples (kSampleBytes each) one launch of these table entries needs.
 inline size_t launch_samples(const WfScene* tab, size_t n)
 {
     size_t s = 0;
@@ -103,7 +103,7 @@ inline void use_hit_list(WfScene& w, void* buf, uint32_t n_hit)
 // sample buffer (in table order) and fills the launch's totals.  False when
 // a count would not fit the kernel's 32-bit indices (a scene's jobs, or the
 // launch's pixels or chunks, reach 2^31).
-inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float4* samples, WfPass* p)
+inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float3* samples, WfPass* p)
 {
     const uint64_t kLimit = 1ull << 31;
     uint64_t chunks = 0, px = 0;
@@ -176,9 +176,9 @@ constexpr uint32_t kDefaultFramesPerLaunch = 64;
 // takes as few launches as its sample buffer allows: 1080p, 256 frames per
 // launch, 0.172 -> 0.160 ms per frame against 64.
 constexpr uint32_t kRendererFramesPerLaunch = 256;
-// Upper bound of one launch's sample buffer (16 B per pixel and frame):
+// Upper bound of one launch's sample buffer (kSampleBytes per pixel and frame):
 // renderers shorten their launches, batches split them by scene groups.
-// 8 GiB is 3 % of the device's HBM and holds 256 frames of 1080p.
+// 8 GiB is 3 % of the device's HBM and holds 345 frames of 1080p.
 constexpr size_t kSampleBufferBytes = size_t(8) << 30;
 
 }  // namespace atx

@@ -115,13 +115,13 @@ def test_wavefront_equals_plain_on_full_1080p():
     assert out[ATX_KERNEL_PLAIN][2] == out[ATX_KERNEL_WAVEFRONT][2]
 
 
-@pytest.mark.parametrize("size,frames,launches", [((48, 30), 600, 3), ((4096, 4096), 70, 3)],
+@pytest.mark.parametrize("size,frames,launches", [((48, 30), 600, 3), ((4096, 4096), 90, 3)],
                          ids=["256-frame-launches", "8GiB-sample-cap"])
 def test_wavefront_launch_splits_equal_plain(size, frames, launches):
     """600 frames split into 256 + 256 + 88 frames per launch, and a
-    4096x4096 image whose 8 GiB sample buffer holds 32 frames (32 + 32 + 6):
-    both must match the plain kernel's frame-by-frame accumulation bit for
-    bit."""
+    4096x4096 image whose 8 GiB sample buffer holds 42 frames of 12-byte
+    samples (42 + 42 + 6): both must match the plain kernel's frame-by-frame
+    accumulation bit for bit."""
     w, h = size
     sc = scene_from_camera_overrides(plain_cpu_scene(), w, h)
     sc.settings = Settings(True, False, 4)

@@ -53,12 +53,12 @@ def test_server_staggered_arrivals_and_budget():
     ref, _ = serve_all(scenes, frames_per_tick=1)
     # admit roughly one request at a time: budget just above the largest request
     # (accumulation + image 20 B/px, primary ray directions 16 B/px, one frame
-    # of samples 16 B/px, first-bounce cache 64 B/px + 16 B/px per light + the
+    # of samples 12 B/px, first-bounce cache 64 B/px + 16 B/px per light + the
     # 4 B/px hit list, scene and tree)
     def cache(s):
         lights = len(s.lights)
         return 68 + (16 * lights if 1 <= lights <= 32 else 0) if s.spp >= 2 else 0
-    budget = max(s.camera.width * s.camera.height * (52 + cache(s)) for s in scenes) + 4096 * 64 + (1 << 17)
+    budget = max(s.camera.width * s.camera.height * (48 + cache(s)) for s in scenes) + 4096 * 64 + (1 << 17)
     res, st = serve_all(scenes, delays=[0.0, 0.01, 0.0, 0.02, 0.0, 0.01, 0.0], frames_per_tick=1,
                         hbm_budget_bytes=budget)
     assert st["hbm_high_water"] <= budget

@@ -32,7 +32,7 @@ static atx::WfScene scene(uint32_t w, uint32_t h, uint32_t frames, uint32_t node
 int main()
 {
     using namespace atx;
-    float4* base = reinterpret_cast<float4*>(uintptr_t(1) << 20);
+    float3* base = reinterpret_cast<float3*>(uintptr_t(1) << 20);
 
     // two scenes: 100x50 px x 3 frames (15000 jobs), 7x3 px x 64 frames (1344 jobs)
     std::vector<WfScene> t = {scene(100, 50, 3, 9), scene(7, 3, 64, 0)};
