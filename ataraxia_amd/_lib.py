@@ -73,7 +73,7 @@ class SceneDesc(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("chunk_px", C.c_uint32), ("workgroups", C.c_uint32), ("use_graph", C.c_uint32),
-                ("frames_per_launch", C.c_uint32)]
+                ("frames_per_launch", C.c_uint32), ("waves_per_simd", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

@@ -67,14 +67,14 @@ class Batch:
         check(lib.atx_batch_set_accel(self._h, accel))
 
     def set_tuning(self, chunk_px: int = 0, workgroups: int = 0, use_graph: bool = True,
-                   frames_per_launch: int = 0) -> None:
-        t = _lib.BatchTuning(chunk_px, workgroups, 1 if use_graph else 0, frames_per_launch)
+                   frames_per_launch: int = 0, waves_per_simd: int = 0) -> None:
+        t = _lib.BatchTuning(chunk_px, workgroups, 1 if use_graph else 0, frames_per_launch, waves_per_simd)
         check(lib.atx_batch_set_tuning(self._h, C.byref(t)))
 
     @staticmethod
     def _tuning(t: _lib.BatchTuning) -> dict:
         return {"chunk_px": t.chunk_px, "workgroups": t.workgroups, "use_graph": bool(t.use_graph),
-                "frames_per_launch": t.frames_per_launch}
+                "frames_per_launch": t.frames_per_launch, "waves_per_simd": t.waves_per_simd}
 
     def get_tuning(self) -> dict:
         t = _lib.BatchTuning()

@@ -1,13 +1,17 @@
-// autotune.cpp -- picks the batch renderer's chunk size and persistent grid by
-// timing candidates on the batch itself, and caches the choice.
+// autotune.cpp -- picks the batch renderer's chunk size, frames per launch,
+// launch bounds (the path kernel's register budget: resident waves per
+// SIMD) and persistent grid by timing candidates on the batch itself, and
+// caches the choice.
 //
 // Candidates: chunk_px in {256, 512} x frames per launch in {32, 64, 128, 256}
-// on the full resident-workgroup grid, then the winner on 3/4 of the grid.
+// with the default kernel instance on its full resident-workgroup grid; then
+// the winner with each other instance (5 to 8 waves per SIMD) on that
+// instance's full grid; then the winner on 3/4 of its grid.
 // Each candidate renders the first F frames of a spread-out subsample of the
 // batch's scenes (up to 16 Mpixel, at least 8 scenes) twice (the second run is
 // timed with HIP events; the graph is re-captured per candidate) and the one
 // with the highest samples/ms wins.  The cache is a text file of
-// "<key>\t<chunk_px>\t<workgroups>\t<frames_per_launch>" lines; the key
+// "<key>\t<chunk_px>\t<workgroups>\t<frames_per_launch>\t<waves_per_simd>" lines; the key
 // names the device (arch, CU count) and the workload (scene count, pixels,
 // spheres, spp, bounces).
 #include <stdio.h>
@@ -64,9 +68,10 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
         while (std::getline(in, line)) {
             std::istringstream ls(line);
             std::string k;
-            unsigned chunk = 0, wg = 0, fpl = 0;
-            if (std::getline(ls, k, '\t') && (ls >> chunk >> wg >> fpl) && k == key && chunk % 64 == 0 && chunk) {
-                atx_batch_tuning t{chunk, wg, 1, fpl};
+            unsigned chunk = 0, wg = 0, fpl = 0, waves = 0;
+            if (std::getline(ls, k, '\t') && (ls >> chunk >> wg >> fpl >> waves) && k == key && chunk % 64 == 0 &&
+                chunk) {
+                atx_batch_tuning t{chunk, wg, 1, fpl, waves};
                 rc = atx_batch_set_tuning(h, &t);
                 if (rc != ATX_OK) return rc;
                 if (chosen) *chosen = t;
@@ -95,7 +100,7 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
     }
     batch_set_subset(b, sample);
     double best_rate = -1.0;
-    atx_batch_tuning best{0, 0, 1, kDefaultFramesPerLaunch};
+    atx_batch_tuning best{0, 0, 1, kDefaultFramesPerLaunch, 0};
     auto try_one = [&](const atx_batch_tuning& t) -> int {
         atx_batch_tuning tt = t;
         int e = atx_batch_set_tuning(h, &tt);
@@ -113,20 +118,30 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
         }
         return ATX_OK;
     };
+    const bool flat = batch_flat_only(b);
+    const uint32_t w0 = wf_waves(0, flat);
     const uint32_t chunks[] = {256u, 512u};
     const uint32_t frames[] = {32u, 64u, 128u, 256u};
     for (uint32_t c : chunks) {
         for (uint32_t F : frames) {
-            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(batch_flat_only(b)), 1, F});
+            rc = try_one(atx_batch_tuning{c, wavefront_full_grid(flat, w0), 1, F, w0});
             if (rc != ATX_OK) {
                 batch_set_subset(b, {});
                 return rc;
             }
         }
     }
+    for (uint32_t w = kWfWavesMin; w <= kWfWavesMax; ++w) {
+        if (w == w0) continue;
+        rc = try_one(atx_batch_tuning{best.chunk_px, wavefront_full_grid(flat, w), 1, best.frames_per_launch, w});
+        if (rc != ATX_OK) {
+            batch_set_subset(b, {});
+            return rc;
+        }
+    }
     const uint32_t three_quarters = (best.workgroups * 3) / 4;
     if (three_quarters > 0) {
-        rc = try_one(atx_batch_tuning{best.chunk_px, three_quarters, 1, best.frames_per_launch});
+        rc = try_one(atx_batch_tuning{best.chunk_px, three_quarters, 1, best.frames_per_launch, best.waves_per_simd});
         if (rc != ATX_OK) {
             batch_set_subset(b, {});
             return rc;
@@ -142,7 +157,8 @@ extern "C" int atx_batch_autotune(atx_batch* h, const char* cache_path, atx_batc
             set_error("cannot write autotune cache %s", cache_path);
             return ATX_ERR_IO;
         }
-        out << key << '\t' << best.chunk_px << '\t' << best.workgroups << '\t' << best.frames_per_launch << '\n';
+        out << key << '\t' << best.chunk_px << '\t' << best.workgroups << '\t' << best.frames_per_launch << '\t'
+            << best.waves_per_simd << '\n';
     }
     return ATX_OK;
 }

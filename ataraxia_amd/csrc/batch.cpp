@@ -46,7 +46,7 @@ struct Batch {
     std::vector<std::unique_ptr<BatchEntry>> scenes;
     int accumulator = ATX_ACCUM_NAIVE;
     int accel = ATX_ACCEL_BVH;
-    atx_batch_tuning tuning{0, 0, 1, 0};
+    atx_batch_tuning tuning{0, 0, 1, 0, 0};
     DevBuf d_table, d_queue, counters;
     std::vector<uint32_t> order;          // table position -> scene id
     std::vector<uint32_t> launch_off;     // first table entry of launch l
@@ -87,7 +87,7 @@ static int enqueue_launches(Batch* b)
         wp.total_px = b->launch_px[l];
         wp.queue = b->d_queue.as<unsigned int>() + l;
         wp.counters = b->counters.as<unsigned long long>();
-        ATX_HIP(launch_wavefront(wp, WfLaunch{b->grid, b->flat_only}, b->stream));
+        ATX_HIP(launch_wavefront(wp, WfLaunch{b->grid, b->flat_only, b->tuning.waves_per_simd}, b->stream));
     }
     return ATX_OK;
 }
@@ -110,7 +110,7 @@ static int batch_prepare(Batch* b, uint32_t max_frames)
     b->frames_per_launch = b->tuning.frames_per_launch ? b->tuning.frames_per_launch : kDefaultFramesPerLaunch;
     b->kahan = b->accumulator == ATX_ACCUM_KAHAN;
     b->flat_only = batch_flat_only(b);
-    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid(b->flat_only);
+    b->grid = b->tuning.workgroups ? b->tuning.workgroups : wavefront_full_grid(b->flat_only, b->tuning.waves_per_simd);
     b->max_spp = 0;
     for (auto& e : b->scenes) b->max_spp = std::max(b->max_spp, std::min(e->spp, max_frames));
     const uint32_t F = b->frames_per_launch;
@@ -402,6 +402,8 @@ int atx_batch_set_tuning(atx_batch* h, const atx_batch_tuning* t)
                   "chunk_px must be a multiple of 64, at most 1024");
     ATX_CHECK_ARG(t->frames_per_launch <= 4096, "frames_per_launch must be <= 4096");
     ATX_CHECK_ARG(t->workgroups <= 65536, "workgroups too large");
+    ATX_CHECK_ARG(t->waves_per_simd == 0 || (t->waves_per_simd >= kWfWavesMin && t->waves_per_simd <= kWfWavesMax),
+                  "waves_per_simd must be 0 or %u..%u", kWfWavesMin, kWfWavesMax);
     b->tuning = *t;
     b->dirty = true;
     return ATX_OK;
@@ -413,8 +415,10 @@ int atx_batch_get_tuning(const atx_batch* h, atx_batch_tuning* t)
     ATX_CHECK_ARG(b && t, "NULL argument");
     *t = b->tuning;
     if (!t->frames_per_launch) t->frames_per_launch = kDefaultFramesPerLaunch;
+    const bool flat = batch_flat_only(b);
+    t->waves_per_simd = wf_waves(t->waves_per_simd, flat);
     if (!t->workgroups)
-        t->workgroups = b->grid ? b->grid : wavefront_full_grid(batch_flat_only(b));
+        t->workgroups = b->grid ? b->grid : wavefront_full_grid(flat, t->waves_per_simd);
     return ATX_OK;
 }
 

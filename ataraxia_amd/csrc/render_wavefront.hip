@@ -660,10 +660,14 @@ hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stre
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
     // no jobs (every pixel's primary ray misses): k_accumulate alone
-    if (p.total_chunks > 0 && l.flat_only)
-        hipLaunchKernelGGL(k_render_wavefront<kWfWavesFlat>, dim3(l.workgroups), dim3(256), 0, stream, p);
-    else if (p.total_chunks > 0)
-        hipLaunchKernelGGL(k_render_wavefront<kWfWavesTree>, dim3(l.workgroups), dim3(256), 0, stream, p);
+    if (p.total_chunks > 0) {
+        switch (wf_waves(l.waves, l.flat_only)) {
+        case 5: hipLaunchKernelGGL(k_render_wavefront<5>, dim3(l.workgroups), dim3(256), 0, stream, p); break;
+        case 6: hipLaunchKernelGGL(k_render_wavefront<6>, dim3(l.workgroups), dim3(256), 0, stream, p); break;
+        case 7: hipLaunchKernelGGL(k_render_wavefront<7>, dim3(l.workgroups), dim3(256), 0, stream, p); break;
+        default: hipLaunchKernelGGL(k_render_wavefront<8>, dim3(l.workgroups), dim3(256), 0, stream, p); break;
+        }
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && paths_done)
         e = hipEventRecord(paths_done, stream);
@@ -718,12 +722,16 @@ hipError_t launch_zero_u32(unsigned int* p, uint32_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-int wavefront_max_blocks_per_cu(bool flat_only)
+int wavefront_max_blocks_per_cu(bool flat_only, uint32_t waves)
 {
     int blocks = 0;
-    const hipError_t e =
-        flat_only ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesFlat>, 256, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<kWfWavesTree>, 256, 0);
+    hipError_t e;
+    switch (wf_waves(waves, flat_only)) {
+    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<5>, 256, 0); break;
+    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<6>, 256, 0); break;
+    case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<7>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_wavefront<8>, 256, 0); break;
+    }
     if (e != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;

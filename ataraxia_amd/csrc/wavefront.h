@@ -109,9 +109,20 @@ __host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pi
 constexpr uint32_t kWfWavesTree = 6;
 constexpr uint32_t kWfWavesFlat = 7;
 
+// Register budgets the path kernel is compiled for (resident waves per
+// SIMD); a launch picks one (the batch autotuner tries them all).
+constexpr uint32_t kWfWavesMin = 5, kWfWavesMax = 8;
+// The instance a launch uses: `waves` when it names a compiled instance,
+// else the default for its scenes.
+__host__ __device__ __forceinline__ uint32_t wf_waves(uint32_t waves, bool flat_only)
+{
+    return waves >= kWfWavesMin && waves <= kWfWavesMax ? waves : (flat_only ? kWfWavesFlat : kWfWavesTree);
+}
+
 struct WfLaunch {
     uint32_t workgroups;     // persistent grid size (256-lane workgroups)
     bool flat_only = false;  // no scene of the launch has a tree
+    uint32_t waves = 0;      // kernel instance (resident waves per SIMD); 0 = default
 };
 
 // The launch's path kernel, then its accumulate kernel; `paths_done`, when
@@ -175,7 +186,8 @@ inline void set_first_cache(WfScene& s, void* buf)
 // Writes the cache s.first points to (s.dirs must hold the primary
 // directions), the hit list included.
 hipError_t launch_first_hits(const WfScene& s, hipStream_t stream);
-// Max resident 256-lane workgroups of the path kernel per CU.
-int wavefront_max_blocks_per_cu(bool flat_only);
+// Max resident 256-lane workgroups of the path kernel per CU (the instance
+// wf_waves(waves, flat_only)).
+int wavefront_max_blocks_per_cu(bool flat_only, uint32_t waves = 0);
 
 }  // namespace atx

@@ -139,9 +139,9 @@ inline uint32_t device_cus()
 }
 
 // Persistent grid that fills the device: CUs x resident workgroups per CU.
-inline uint32_t wavefront_full_grid(bool flat_only)
+inline uint32_t wavefront_full_grid(bool flat_only, uint32_t waves = 0)
 {
-    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu(flat_only);
+    return device_cus() * (uint32_t)wavefront_max_blocks_per_cu(flat_only, waves);
 }
 
 // True when no scene of the table has a tree (the flat-loop kernel instance).

@@ -223,6 +223,9 @@ typedef struct atx_batch_tuning {
     uint32_t workgroups;        /* persistent grid, 256-lane workgroups (0 = fill device) */
     uint32_t use_graph;         /* capture all launches in a HIP graph and replay it       */
     uint32_t frames_per_launch; /* frames each launch renders per scene (0 = 64)           */
+    uint32_t waves_per_simd;    /* the path kernel's register budget: resident waves per
+                                   SIMD it is compiled for, 5..8 (0 = 6 with trees, 7 for
+                                   flat loops only); the launch bounds the autotuner picks */
 } atx_batch_tuning;
 
 typedef struct atx_batch atx_batch;

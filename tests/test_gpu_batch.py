@@ -72,7 +72,10 @@ def test_batch_kahan_matches_oracle():
     {"chunk_px": 64, "workgroups": 1, "use_graph": False, "frames_per_launch": 2},
     {"chunk_px": 256, "workgroups": 0, "use_graph": True, "frames_per_launch": 64},
     {"chunk_px": 192, "workgroups": 37, "use_graph": False, "frames_per_launch": 3},
-], ids=["c64-g2-f1", "c64-g1-f2-eager", "c256-full-f64", "c192-g37-f3-eager"])
+    {"chunk_px": 256, "workgroups": 0, "use_graph": True, "frames_per_launch": 4, "waves_per_simd": 5},
+    {"chunk_px": 128, "workgroups": 0, "use_graph": True, "frames_per_launch": 2, "waves_per_simd": 7},
+    {"chunk_px": 512, "workgroups": 0, "use_graph": False, "frames_per_launch": 8, "waves_per_simd": 8},
+], ids=["c64-g2-f1", "c64-g1-f2-eager", "c256-full-f64", "c192-g37-f3-eager", "w5", "w7", "w8-eager"])
 def test_batch_independent_of_tuning(tuning):
     scenes = mixed_scenes()
     print("case tuning", tuning, flush=True)
@@ -216,12 +219,12 @@ def test_autotune_picks_valid_tuning_and_caches_it(tmp_path):
         ids = [b.add_scene(s) for s in scenes]
         t1 = b.autotune(str(cache))
         assert t1["chunk_px"] in (256, 512) and t1["frames_per_launch"] in (32, 64, 128, 256)
-        assert t1["workgroups"] > 0
+        assert t1["workgroups"] > 0 and t1["waves_per_simd"] in (5, 6, 7, 8)
         b.render()
         accs = [b.get_accumulation(i).reshape(-1, 4).copy() for i in ids]
     lines = cache.read_text().splitlines()
     assert len(lines) == 1 and lines[0].split("\t")[1:] == [str(t1["chunk_px"]), str(t1["workgroups"]),
-                                                           str(t1["frames_per_launch"])]
+                                                           str(t1["frames_per_launch"]), str(t1["waves_per_simd"])]
     with Batch() as b:
         for s in scenes:
             b.add_scene(s)
