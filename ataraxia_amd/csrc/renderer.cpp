@@ -44,6 +44,7 @@ struct Renderer {
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kRendererFramesPerLaunch;
+    uint32_t wf_waves = 0;   // path-kernel instance (resident waves per SIMD); 0: default
     DevBuf accum, comp, image, counters;
     uint32_t width = 0, height = 0;
     atx_settings settings{1, 0, {0, 0}, 15};  // Scene.h:49-56 defaults
@@ -140,7 +141,8 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         for (const WfScene& e : table) most = std::max(most, launch_samples(&e, 1));
         ATX_HIP(r->wf_samples.reserve(most * kSampleBytes));
         shape.flat_only = wf_flat_only(table.data(), table.size());
-        shape.workgroups = wavefront_full_grid(shape.flat_only);
+        shape.waves = r->wf_waves;
+        shape.workgroups = wavefront_full_grid(shape.flat_only, shape.waves);
         // The first-bounce cache pays from two frames per camera on: a call
         // of one frame uses it once the camera has stayed the same since the
         // previous call (a camera that moves every frame never builds it).
@@ -279,6 +281,10 @@ int atx_renderer_create(atx_renderer** out)
     if (const char* env = getenv("ATX_WF_CHUNK")) {
         const uint32_t c = (uint32_t)strtoul(env, nullptr, 10);
         if (c >= 64 && c % 64 == 0 && c <= kMaxChunkPx) r->wf_chunk = c;
+    }
+    if (const char* env = getenv("ATX_WF_WAVES")) {
+        const uint32_t w = (uint32_t)strtoul(env, nullptr, 10);
+        if (w >= kWfWavesMin && w <= kWfWavesMax) r->wf_waves = w;
     }
     if (const char* env = getenv("ATX_WF_FRAMES")) {
         const uint32_t f = (uint32_t)strtoul(env, nullptr, 10);

@@ -67,6 +67,8 @@ _PROTOS = {
                                       _P, _P, _P, _U32, _P, _U32, _U32]),
     "ora_accumulate_pixels": (C.c_int, [C.POINTER(OraCamera), C.POINTER(OraScene), C.POINTER(OraSettings), _U32,
                                         _U32, _P, _U64, C.c_int, _P, _P, C.c_int]),
+    "ora_accumulate_pixels_f64": (C.c_int, [C.POINTER(OraCamera), C.POINTER(OraScene), C.POINTER(OraSettings),
+                                            _U32, _U32, _P, _U64, _P, C.c_int]),
     "ora_node_local_transform": (None, [_P, _P, _P, _P]),
     "ora_mat4_mul": (None, [_P, _P, _P]),
     "ora_flatten_sphere": (None, [_P, _P, _P]),
@@ -230,6 +232,19 @@ def accumulate_pixels(cam: OraCamera, scene: Scene, sett: OraSettings, first_fra
         if rc != 0:
             raise RuntimeError("oracle thread creation failed")
     return acc, img
+
+
+def accumulate_pixels_f64(cam: OraCamera, scene: Scene, sett: OraSettings, first_frame: int, frames: int, pixels,
+                          threads: int = 8):
+    """The float64 sum of the same float32 samples (4 doubles per listed pixel)."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint32)
+    acc = np.zeros((len(px), 4), np.float64)
+    if len(px):
+        rc = lib.ora_accumulate_pixels_f64(C.byref(cam), C.byref(scene.c), C.byref(sett), first_frame, frames,
+                                           _p(px), len(px), _p(acc), threads)
+        if rc != 0:
+            raise RuntimeError("oracle thread creation failed")
+    return acc
 
 
 def pcg_hash(seed: int) -> int:

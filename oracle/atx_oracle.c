@@ -718,6 +718,7 @@ typedef struct {
     const float* ip; const float* iv;
     uint32_t first_frame, frames; const uint32_t* pixels; uint64_t n; int kahan;
     float* accum; uint32_t* image; volatile uint64_t* next;
+    double* accum64;   /* float64 sums instead (accum and image unused) */
 } px_job;
 
 static void* px_worker(void* arg)
@@ -730,6 +731,7 @@ static void* px_worker(void* arg)
             break;
         uint32_t x = j->pixels[i] % W, y = j->pixels[i] / W;
         float d[3], acc[4] = {0, 0, 0, 0}, cp[4] = {0, 0, 0, 0};
+        double a64[4] = {0, 0, 0, 0};
         ora_ray_direction(j->ip, j->iv, x, y, W, j->cam->height, d);
         /* Renderer.cu:245-248: the index advances with accumulation, else stays 1 */
         uint32_t fi = j->first_frame;
@@ -737,9 +739,11 @@ static void* px_worker(void* arg)
             float c[4];
             ora_per_pixel(x, y, W, d, j->cam->position, j->sc, fi, j->settings, c, 0);
             if (fi == 1u)
-                for (int k = 0; k < 4; ++k) acc[k] = cp[k] = 0.0f;
+                for (int k = 0; k < 4; ++k) { acc[k] = cp[k] = 0.0f; a64[k] = 0.0; }
             for (int k = 0; k < 4; ++k) {
-                if (j->kahan) {
+                if (j->accum64) {
+                    a64[k] += (double)c[k];
+                } else if (j->kahan) {
                     float yk = c[k] - cp[k];
                     float t = acc[k] + yk;
                     cp[k] = (t - acc[k]) - yk;
@@ -750,6 +754,10 @@ static void* px_worker(void* arg)
             }
             if (f + 1 < j->frames)
                 fi = j->settings->accumulation ? fi + 1u : 1u;
+        }
+        if (j->accum64) {
+            memcpy(j->accum64 + 4 * i, a64, sizeof(a64));
+            continue;
         }
         memcpy(j->accum + 4 * i, acc, sizeof(acc));
         if (j->image) {
@@ -762,9 +770,9 @@ static void* px_worker(void* arg)
     return 0;
 }
 
-int ora_accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_settings* settings,
-                          uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
-                          int kahan, float* accum, uint32_t* image, int threads)
+static int accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_settings* settings,
+                             uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                             int kahan, float* accum, uint32_t* image, double* accum64, int threads)
 {
     float ip[16], iv[16];
     ora_camera_matrices(cam, ip, iv);
@@ -772,7 +780,7 @@ int ora_accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_
     if (threads > 256) threads = 256;
     volatile uint64_t next = 0;
     pthread_t tid[256];
-    px_job job = {cam, sc, settings, ip, iv, first_frame, frames, pixels, n, kahan, accum, image, &next};
+    px_job job = {cam, sc, settings, ip, iv, first_frame, frames, pixels, n, kahan, accum, image, &next, accum64};
     for (int t = 0; t < threads; ++t)
         if (pthread_create(&tid[t], 0, px_worker, &job) != 0) {
             for (int u = 0; u < t; ++u) pthread_join(tid[u], 0);
@@ -781,6 +789,20 @@ int ora_accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_
     for (int t = 0; t < threads; ++t)
         pthread_join(tid[t], 0);
     return 0;
+}
+
+int ora_accumulate_pixels(const ora_camera* cam, const ora_scene* sc, const ora_settings* settings,
+                          uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                          int kahan, float* accum, uint32_t* image, int threads)
+{
+    return accumulate_pixels(cam, sc, settings, first_frame, frames, pixels, n, kahan, accum, image, 0, threads);
+}
+
+int ora_accumulate_pixels_f64(const ora_camera* cam, const ora_scene* sc, const ora_settings* settings,
+                              uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                              double* accum, int threads)
+{
+    return accumulate_pixels(cam, sc, settings, first_frame, frames, pixels, n, 0, 0, 0, accum, threads);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -128,10 +128,15 @@ uint32_t ora_rgba8(const float rgba[4]);
  * as Renderer::Render leaves it: accum[4*i..] for pixels[i], the naive sum
  * of each frame's sample, or the Kahan-compensated one when kahan != 0.
  * image (optional) gets RGBA8 of accum / last frame index.  Runs on
- * `threads` POSIX threads over the pixel list.                            */
+ * `threads` POSIX threads over the pixel list.
+ * ora_accumulate_pixels_f64 sums the same float32 samples in float64 (the
+ * precision reference; accum holds 4 doubles per pixel).                  */
 int  ora_accumulate_pixels(const ora_camera* cam, const ora_scene* scene, const ora_settings* settings,
                            uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
                            int kahan, float* accum, uint32_t* image, int threads);
+int  ora_accumulate_pixels_f64(const ora_camera* cam, const ora_scene* scene, const ora_settings* settings,
+                               uint32_t first_frame, uint32_t frames, const uint32_t* pixels, uint64_t n,
+                               double* accum, int threads);
 
 /* Per-pixel debug capture of one frame over rows [row0,row1): radiance (4
  * floats per pixel), continuation and shadow hit ids per bounce (pixel-major,

@@ -343,6 +343,11 @@ def test_accumulate_pixels_equals_frame_by_frame():
             acc, comp, img, _, _ = O.render_frame(cam, osc, st, f, accum=acc, comp=comp)
         got, gimg = O.accumulate_pixels(cam, osc, st, 1, 5, pix, kahan=kahan, threads=3)
         assert np.array_equal(got, acc[pix]) and np.array_equal(gimg, img[pix])
+        # the float64 sum of the same float32 samples
+        s64 = np.zeros((n, 4), np.float64)
+        for f in range(1, 6):
+            s64 += O.render_frame(cam, osc, st, f)[3]
+        assert np.array_equal(O.accumulate_pixels_f64(cam, osc, st, 1, 5, pix, threads=2), s64[pix])
         # accumulation off: the index stays 1 and every frame restarts the sum
         st = O.settings(5, False, False)
         one, _, img1, _, _ = O.render_frame(cam, osc, st, 1, accum=np.zeros((n, 4), np.float32),
