@@ -53,6 +53,8 @@
 #include "trace_device.h"
 #include "wavefront.h"
 
+#include <stdlib.h>
+
 namespace atx {
 namespace {
 
@@ -253,6 +255,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         for (;;) {
             const unsigned long long idle = __ballot(state == kIdle);
             if (idle == 0ull)
+                break;
+            // Jobs are handed out once refill_min lanes wait (or all do):
+            // the hand-out and bounce 0 from the cache (about a sixth of the
+            // kernel's time) then run for many lanes at once instead of for
+            // the one or two whose paths just ended, while the waiting lanes
+            // cost the trace loop little (its length is set by its slowest
+            // lane).  1080p, A/B: 16 lanes 35.0 -> 34.0 ms per 256-frame
+            // launch, 3.72 -> 3.44 ms per 20-frame one.
+            if (idle != ~0ull && (uint32_t)__popcll(idle) < a.refill_min && blk_next < blk_end)
                 break;
             if (blk_next < blk_end) {
                 const uint32_t avail = blk_end - blk_next;
@@ -653,10 +664,17 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n)
 
 }  // namespace
 
-hipError_t launch_wavefront(const WfPass& p, const WfLaunch& l, hipStream_t stream, hipEvent_t paths_done)
+hipError_t launch_wavefront(const WfPass& pass, const WfLaunch& l, hipStream_t stream, hipEvent_t paths_done)
 {
-    if (p.total_px == 0)
+    if (pass.total_px == 0)
         return hipSuccess;
+    WfPass p = pass;
+    static const uint32_t refill = [] {
+        const char* env = getenv("ATX_WF_REFILL");
+        const long v = env ? strtol(env, nullptr, 10) : (long)kRefillMin;
+        return (uint32_t)(v < 1 ? 1 : v > 64 ? 64 : v);
+    }();
+    p.refill_min = refill;
     if (p.chunk_px == 0 || p.chunk_px % 64 != 0 || p.chunk_px > kMaxChunkPx)
         return hipErrorInvalidValue;
     // no jobs (every pixel's primary ray misses): k_accumulate alone

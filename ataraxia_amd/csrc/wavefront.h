@@ -78,7 +78,13 @@ struct WfPass {
     uint32_t total_px;     // pixels of all scenes of the launch
     unsigned int* queue;   // this launch's chunk counter (zero before launch)
     unsigned long long* counters;
+    uint32_t refill_min;   // idle lanes a wave waits for before it hands out jobs (set by launch_wavefront)
+    uint32_t pad_;
 };
+
+// Idle lanes a wave of the path kernel collects before it hands out new
+// jobs (unless every lane is idle); ATX_WF_REFILL (1-64) overrides it.
+constexpr uint32_t kRefillMin = 16;
 
 constexpr uint32_t kMaxChunkPx = 1024;
 
