@@ -385,10 +385,12 @@ def bench_precision(args):
     r.set_scene(sc)
     r.set_accumulator(ATX_ACCUM_KAHAN)
     t0 = time.perf_counter()
-    for _ in range(spp):
+    for f in range(spp):
         rad, _, _ = r.render_debug(cam, 1)
         s64 += rad
         s32 = s32 + rad
+        if (f + 1) % 512 == 0:
+            print(f"precision: {f + 1}/{spp} frames, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     kah = r.get_accumulation().reshape(n, 4)
     r.close()
@@ -402,7 +404,7 @@ def bench_precision(args):
             "unit": "relative error (max over pixels and channels)", "n_gpus": 1, "steps": spp, "warmup": 0,
             "ms_per_step": wall * 1e3 / spp, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic (seeded splitmix64 scene generator)",
-            "config": {"workload": f"precision: {side}x{side} (of 1024x1024), 512 spheres + ground, 4 lights, "
+            "config": {"workload": f"precision: {side}x{side} of the 1024x1024 scene, 512 spheres + ground, 4 lights, "
                                    f"6 bounces, {spp} spp"},
             "kahan": ek, "naive": en, "kahan_below_1e-6": ek["max"] < 1e-6}
 
