@@ -24,9 +24,9 @@
 // (job_pixel_frame), so a wave traces 8 neighbouring pixels in many frames
 // at once: their rays start from the same few hits, which keeps the lanes of
 // a wave on nearby tree nodes and makes whole waves miss the same spheres in
-// the flat loop.  Samples are stored frame-major ([frame][pixel]), so that
-// k_accumulate reads them coalesced, and the 8 lanes that run one frame of a
-// group write one whole 128-byte line together.
+// the flat loop.  Samples are stored in job order: a wave's block of jobs
+// fills whole 128-byte lines of the sample buffer, which no other wave
+// touches, and k_accumulate reads a pixel's frames 8 samples apart.
 //
 // First-bounce cache: a pixel's primary ray, the shading at its hit and the
 // shadow ray towards each light are the same in every frame.  When the
@@ -208,12 +208,17 @@ __device__ __forceinline__ bool next_bounce(CScene* S, int32_t mid, f3 nrm, f3 s
     return false;
 }
 
-// The path's sample (and the last frame's debug capture).
-__device__ __forceinline__ void store_sample(CScene* S, uint32_t frame, uint32_t px, f3 col)
+// The sample of the scene's job `job` (pixel px), and the last frame's
+// debug capture.
+__device__ __forceinline__ void store_sample(CScene* S, uint32_t job, uint32_t px, f3 col)
 {
-    S->samples[(size_t)(frame - S->frame0) * (S->width * S->height) + px] = make_float3(col.x, col.y, col.z);
-    if (S->dbg_radiance && frame == S->frame0 + S->frames - 1u)
-        S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
+    S->samples[job] = make_float3(col.x, col.y, col.z);
+    if (S->dbg_radiance) {
+        uint32_t slot, f;
+        job_pixel_frame(job, S->hit_px ? S->n_hit : S->width * S->height, S->frames, slot, f);
+        if (f == S->frames - 1u)
+            S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
+    }
 }
 
 template <uint32_t kWaves>
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
 
     // ---- per-lane path state ----
     uint32_t state = kIdle;
-    uint32_t px = 0, frame = 0, seed = 0;
+    uint32_t px = 0, job = 0, seed = 0;   // pixel, the scene's job index, PCG state
     int32_t bounce = 0;
     f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, tp = ro, col = ro;
     f3 so = ro, nrm = ro, L = ro;   // shadow/next origin, normal, light direction
@@ -270,8 +275,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
                 const uint32_t nidle = (uint32_t)__popcll(idle);
                 if (state == kIdle && rank < avail) {
-                    const uint32_t j = blk0 + blk_next + rank;
-                    job_pixel_frame(j, S->hit_px ? S->n_hit : S->width * S->height, S->frames, px, frame);
+                    job = blk0 + blk_next + rank;
+                    uint32_t frame;
+                    job_pixel_frame(job, S->hit_px ? S->n_hit : S->width * S->height, S->frames, px, frame);
                     if (S->hit_px)
                         px = S->hit_px[px];
                     frame += S->frame0;
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                             done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
                         }
                         if (done) {
-                            store_sample(S, frame, px, col);
+                            store_sample(S, job, px, col);
                             state = kIdle;
                         }
                     }
@@ -462,7 +468,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         if (__any(tests >= 0x80000000u || boxes >= 0x80000000u || reused >= 0x80000000u))
             flush_counters(a.counters, lane, tests, boxes, reused);
         if (end_path) {
-            store_sample(S, frame, px, col);
+            store_sample(S, job, px, col);
             state = kIdle;
         }
     }
@@ -475,7 +481,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
 // Adds every pixel's samples of the launch to its accumulation in frame order
 // (Renderer.cu:164-165; frame 1 starts from zero, which is what the
 // reference's memset provides) and writes the RGBA8 value (Renderer.cu:166-168,
-// Renderer.h:70-78).  One lane per pixel of the launch's pixel range.
+// Renderer.h:70-78).  One lane per pixel of the launch's pixel range; a
+// pixel's samples lie at job_of(slot, f), m samples apart from one frame to
+// the next (m = 8 but in a scene's last group of slots).
 __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
 {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
@@ -498,6 +506,13 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
     const bool sky = S.hit_px && (int32_t)S.first[4 * (size_t)px].x < 0;
     const float4 sky_c = S.sky_light ? make_float4(0.6f, 0.7f, 0.9f, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     float4 acc = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.accum[px];
+    // this pixel's samples: frame f at sp[f * step] (slots are hit-list
+    // indices with a hit list, else pixels; a sky pixel has none)
+    const uint32_t slots = S.hit_px ? S.n_hit : npx;
+    const uint32_t slot = sky ? 0u : S.hit_px ? S.first[4 * (size_t)px + 3].y : px;
+    const uint32_t left = slots - (slot & ~(kJobGroupPx - 1u));
+    const uint32_t step = left < kJobGroupPx ? left : kJobGroupPx;
+    const ATX_GLOBAL float3* sp = (const ATX_GLOBAL float3*)(uintptr_t)S.samples + job_of(slot, 0u, slots, S.frames);
     if (S.kahan) {
         float4 cp = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.comp[px];
         const auto add = [&](const float4 c) {
@@ -513,9 +528,8 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
             for (uint32_t f = 0; f < S.frames; ++f)
                 add(sky_c);
         } else {
-            const ATX_GLOBAL float3* sp = (const ATX_GLOBAL float3*)(uintptr_t)S.samples + px;
             for (uint32_t f = 0; f < S.frames; ++f) {
-                const float3 c = sp[(size_t)f * npx];
+                const float3 c = sp[(size_t)f * step];
                 add(make_float4(c.x, c.y, c.z, 1.0f));
             }
         }
@@ -530,12 +544,11 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
     } else {
         // four frames' loads in flight ahead of their additions (in frame
         // order); every sample's alpha is 1
-        const ATX_GLOBAL float3* sp = (const ATX_GLOBAL float3*)(uintptr_t)S.samples + px;
         uint32_t f = 0;
         for (; f + 4u <= S.frames; f += 4u) {
             float3 c[4];
             for (int k = 0; k < 4; ++k)
-                c[k] = sp[(size_t)(f + k) * npx];
+                c[k] = sp[(size_t)(f + k) * step];
             for (int k = 0; k < 4; ++k) {
                 acc.x = acc.x + c[k].x;
                 acc.y = acc.y + c[k].y;
@@ -544,7 +557,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
             }
         }
         for (; f < S.frames; ++f) {
-            const float3 c = sp[(size_t)f * npx];
+            const float3 c = sp[(size_t)f * step];
             acc.x = acc.x + c.x;
             acc.y = acc.y + c.y;
             acc.z = acc.z + c.z;
@@ -607,6 +620,7 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
         return trace_filtered(flat, s.n_sph, o, d, ex);
     };
     const Hit h = trace(ro, rd, -1.0f);
+    uint32_t slot = 0u;   // index in the hit list
     {
         // append this wave's hit pixels to the list (one atomic per wave)
         const uint32_t lane = threadIdx.x & 63u;
@@ -619,7 +633,8 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
         base = (uint32_t)__shfl((int)base, (int)leader, 64);
         if (h.id >= 0) {
             const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
-            a.hits[base + (uint32_t)__popcll(hm & below)] = px;
+            slot = base + (uint32_t)__popcll(hm & below);
+            a.hits[slot] = px;
         }
     }
     const f3 one = mk(1.0f, 1.0f, 1.0f);
@@ -653,7 +668,7 @@ __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
     r[0] = make_uint4((uint32_t)h.id, __float_as_uint(h.t), mask, (uint32_t)mid);
     r[1] = make_uint4(__float_as_uint(nrm.x), __float_as_uint(nrm.y), __float_as_uint(nrm.z), __float_as_uint(so.x));
     r[2] = make_uint4(__float_as_uint(so.y), __float_as_uint(so.z), __float_as_uint(col.x), __float_as_uint(col.y));
-    r[3] = make_uint4(__float_as_uint(col.z), 0u, 0u, 0u);
+    r[3] = make_uint4(__float_as_uint(col.z), slot, 0u, 0u);
 }
 
 __global__ void k_zero_u32(unsigned int* p, uint32_t n)

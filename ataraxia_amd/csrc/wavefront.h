@@ -55,8 +55,8 @@ struct WfScene {
     float4* dbg_radiance;  // optional per-pixel capture of the last frame (nullptr = off)
     int32_t* dbg_hit;
     int32_t* dbg_shadow;
-    float3* samples;       // this launch's samples: [frames][width * height], (r, g, b);
-                           // k_accumulate adds the alpha of 1 itself
+    float3* samples;       // this launch's samples (r, g, b), indexed by the scene's job
+                           // (job_pixel_frame order); k_accumulate adds the alpha of 1 itself
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
     const float4* first_light; // first-bounce light terms, [light][pixel] (nullptr: none)
@@ -96,8 +96,10 @@ constexpr size_t kSampleBytes = 12;
 // after frame inside a group: jobs 8g .. 8g+7 are pixels 8g .. 8g+7 of the
 // first frame, the next 8 the same pixels in the second frame, and so on.
 // The 64 lanes of a wave then run 8 frames of 8 neighbouring pixels at a
-// time, and the 8 samples of one 128-byte line of the [frame][pixel] sample
-// buffer are written together.
+// time.  Samples are stored in job order, so a wave's block of chunk_px
+// consecutive jobs (chunk_px * 12 bytes, a multiple of 128) fills whole
+// 128-byte lines of the sample buffer that no other wave writes.
+// "Pixel" here is a slot: a pixel, or an index into the scene's hit list.
 constexpr uint32_t kJobGroupPx = 8;
 __host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pixels, uint32_t frames, uint32_t& px,
                                                          uint32_t& frame)
@@ -108,6 +110,15 @@ __host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pi
     const uint32_t m = left < kJobGroupPx ? left : kJobGroupPx;
     frame = w / m;
     px = kJobGroupPx * g + (w - frame * m);
+}
+
+// The job of slot k in the launch's frame f (the inverse of job_pixel_frame).
+__host__ __device__ __forceinline__ uint32_t job_of(uint32_t k, uint32_t f, uint32_t pixels, uint32_t frames)
+{
+    const uint32_t g = k / kJobGroupPx;
+    const uint32_t left = pixels - kJobGroupPx * g;
+    const uint32_t m = left < kJobGroupPx ? left : kJobGroupPx;
+    return g * kJobGroupPx * frames + f * m + (k - g * kJobGroupPx);
 }
 
 // Resident waves per SIMD the path kernel's registers are budgeted for:
@@ -150,7 +161,7 @@ hipError_t launch_camera_dirs(const WfScene& s, float4* dirs, hipStream_t stream
 //          lights), the hit's material;
 //      [1] normal x, y, z, origin of the rays leaving the hit x;
 //      [2] origin y, z, radiance after the hit's emission r, g;
-//      [3] radiance b, 0, 0, 0;
+//      [3] radiance b, the pixel's index in the hit list (hits only), 0, 0;
 //  * then, for scenes of 1..kFirstMaxLights lights, per light and pixel
 //    ([light][pixel]) the light's Cook-Torrance term at the hit (float4,
 //    w = 0; zero when the light does not reach the hit);

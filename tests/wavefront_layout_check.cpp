@@ -79,9 +79,25 @@ int main()
             uint32_t px = 0, f = 0;
             job_pixel_frame(j, npx, frames, px, f);
             if (px >= npx || f >= frames || seen[(size_t)f * npx + px]++) { ok = false; break; }
+            // job_of inverts it (samples are stored by job)
+            if (job_of(px, f, npx, frames) != j) { ok = false; break; }
         }
         CHECK(ok);
     }
+    // a pixel's frames lie min(8, pixels left in its group) jobs apart, which
+    // is the stride k_accumulate reads with
+    for (const auto& sz : sizes) {
+        const uint32_t npx = sz[0], frames = sz[1];
+        bool ok = true;
+        for (uint32_t k = 0; k < npx && ok; ++k) {
+            const uint32_t left = npx - (k & ~7u), step = left < 8 ? left : 8;
+            for (uint32_t f = 0; f < frames; ++f)
+                if (job_of(k, f, npx, frames) != job_of(k, 0, npx, frames) + f * step) { ok = false; break; }
+        }
+        CHECK(ok);
+    }
+    // a block of 64 consecutive jobs is 768 contiguous bytes of samples
+    CHECK(64 * kSampleBytes % 128 == 0 && kMaxChunkPx % 64 == 0);
     for (uint32_t j = 0; j < 64; ++j) {
         uint32_t px = 0, f = 0;
         job_pixel_frame(8 * 64 * 3 + j, 1920 * 1080, 64, px, f);
