@@ -160,7 +160,7 @@ __device__ __forceinline__ bool shade_hit(CScene* S, const Hit& h, f3 ro, f3 rd,
     add_emission(S->mat, mid, tp, col);
     if (S->n_light == 0)
         return false;
-    lidx = pcg_hash(seed) % S->n_light;
+    lidx = umod(pcg_hash(seed), S->div_light);   // pcg_hash(seed) % n_light
     light_ray(S->light[2 * lidx], pos, L, dist2);
     return true;
 }
@@ -208,14 +208,14 @@ __device__ __forceinline__ bool next_bounce(CScene* S, int32_t mid, f3 nrm, f3 s
     return false;
 }
 
-// The sample of the scene's job `job` (pixel px), and the last frame's
-// debug capture.
-__device__ __forceinline__ void store_sample(CScene* S, uint32_t job, uint32_t px, f3 col)
+// The sample of the scene's job `job` (pixel px; `slots` jobs per frame),
+// and the last frame's debug capture.
+__device__ __forceinline__ void store_sample(CScene* S, uint32_t slots, uint32_t job, uint32_t px, f3 col)
 {
     S->samples[job] = make_float3(col.x, col.y, col.z);
     if (S->dbg_radiance) {
         uint32_t slot, f;
-        job_pixel_frame(job, S->hit_px ? S->n_hit : S->width * S->height, S->frames, slot, f);
+        job_pixel_frame(job, slots, S->frames, slot, f);
         if (f == S->frames - 1u)
             S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
     }
@@ -238,6 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     const uint32_t n_waves = gridDim.x * 4u;
     const uint32_t wave_id = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
     uint32_t blk0 = 0, blk_next = 0, blk_end = 0;   // jobs left: [blk0 + next, blk0 + end) of scene S
+    uint32_t slots = 0;                             // S's jobs per frame (job_slots)
 
     // ---- per-lane path state ----
     uint32_t state = kIdle;
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 if (state == kIdle && rank < avail) {
                     job = blk0 + blk_next + rank;
                     uint32_t frame;
-                    job_pixel_frame(job, S->hit_px ? S->n_hit : S->width * S->height, S->frames, px, frame);
+                    job_pixel_frame(job, slots, S->div_group, px, frame);
                     if (S->hit_px)
                         px = S->hit_px[px];
                     frame += S->frame0;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         // the frame's light is known before the record arrives: issue
                         // the record's four loads and the light term's load together
                         const uint32_t nl = S->n_light;
-                        const uint32_t lsel = nl > 0u ? pcg_hash(seed) % nl : 0u;
+                        const uint32_t lsel = nl > 0u ? umod(pcg_hash(seed), S->div_light) : 0u;
                         const uint4* fr = S->first + 4 * (size_t)px;
                         const uint4 r0 = fr[0], r1 = fr[1], r2 = fr[2], r3 = fr[3];
                         const float4 lt = S->first_light ? S->first_light[(size_t)lsel * (S->width * S->height) + px]
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                             done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
                         }
                         if (done) {
-                            store_sample(S, job, px, col);
+                            store_sample(S, slots, job, px, col);
                             state = kIdle;
                         }
                     }
@@ -398,12 +399,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 }
                 sid = s;
                 S = &scenes[s];
+                slots = (uint32_t)__builtin_amdgcn_readfirstlane((int)job_slots(S->hit_px, S->width * S->height));
             }
             pend = -1;
-            const uint32_t jobs = (S->hit_px ? S->n_hit : S->width * S->height) * S->frames;
+            const uint32_t jobs = slots * S->frames;
             blk0 = ((uint32_t)c - S->chunk_begin) * P;
             blk_end = blk0 >= jobs ? 0u : (jobs - blk0 < P ? jobs - blk0 : P);
             blk_next = 0;
+            // the host may lay a scene out for more jobs than its hit list
+            // holds: past the last scene's jobs every later chunk is empty
+            if (blk_end == 0u && s + 1 == (int32_t)a.n_scenes)
+                qdone = true;
         }
         if (!__any(state != kIdle) && qdone && pend < 0)
             break;
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         if (__any(tests >= 0x80000000u || boxes >= 0x80000000u || reused >= 0x80000000u))
             flush_counters(a.counters, lane, tests, boxes, reused);
         if (end_path) {
-            store_sample(S, job, px, col);
+            store_sample(S, slots, job, px, col);
             state = kIdle;
         }
     }
@@ -508,7 +514,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfPass a)
     float4 acc = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.accum[px];
     // this pixel's samples: frame f at sp[f * step] (slots are hit-list
     // indices with a hit list, else pixels; a sky pixel has none)
-    const uint32_t slots = S.hit_px ? S.n_hit : npx;
+    const uint32_t slots = job_slots(S.hit_px, npx);
     const uint32_t slot = sky ? 0u : S.hit_px ? S.first[4 * (size_t)px + 3].y : px;
     const uint32_t left = slots - (slot & ~(kJobGroupPx - 1u));
     const uint32_t step = left < kJobGroupPx ? left : kJobGroupPx;

@@ -40,7 +40,10 @@ struct Renderer {
     // cache of that camera and the current scene (set_scene clears first_valid)
     atx_camera dirs_cam{}, prev_cam{};
     bool dirs_valid = false, first_valid = false, prev_valid = false;
-    uint32_t n_hit = 0;   // pixels of wf_first's hit list
+    // pixels of wf_first's hit list, read back after the call that built it
+    // (n_hit_exact: for the current cache; else the previous cache's)
+    uint32_t n_hit = 0;
+    bool n_hit_exact = false;
     DevBuf dbg_rad, dbg_hit, dbg_sh;   // debug capture buffers, reused across calls
     uint32_t wf_chunk = 0;   // jobs per chunk; 0: auto_chunk_px per launch
     uint32_t wf_frames = kRendererFramesPerLaunch;
@@ -163,14 +166,21 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             r->dirs_cam = *cam;
             r->dirs_valid = true;
         }
+        // The path kernel reads the hit list's length on the device, so the
+        // launches follow k_first_hit without the host waiting for it: until
+        // the length is read back (with this call's statistics) the chunks
+        // are laid out for every pixel and their size chosen for the previous
+        // cache's length.
         if (use_first && !r->first_valid) {
             ATX_HIP(launch_first_hits(table[0], r->stream));
-            ATX_HIP(read_first_hits(r->wf_first.get(), npx, r->geo.n_light, r->stream, &r->n_hit));
             r->first_valid = true;
+            r->n_hit_exact = false;
         }
         for (WfScene& e : table) {
-            if (use_first) use_hit_list(e, r->wf_first.get(), r->n_hit);
-            const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(scene_jobs(e), shape.workgroups);
+            if (use_first) use_hit_list(e, r->wf_first.get(), r->n_hit_exact ? r->n_hit : (uint32_t)npx);
+            const uint64_t est =
+                e.hit_px && !r->n_hit_exact && r->n_hit ? (uint64_t)r->n_hit * e.frames : scene_jobs(e);
+            const uint32_t P = r->wf_chunk ? r->wf_chunk : auto_chunk_px(est, shape.workgroups);
             launch_chunk.push_back(P);
             WfPass lp{};
             if (!layout_launch(&e, 1, P, r->wf_samples.as<float3>(), &lp)) {
@@ -222,6 +232,11 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
     ATX_HIP(hipEventRecord(r->ev1, r->stream));
     unsigned long long ctr[kCtrCount];
     ATX_HIP(hipMemcpyAsync(ctr, r->counters.get(), sizeof(ctr), hipMemcpyDeviceToHost, r->stream));
+    const bool read_hits = use_first && !r->n_hit_exact;
+    uint32_t n_hit_read = 0;
+    if (read_hits)
+        ATX_HIP(hipMemcpyAsync(&n_hit_read, first_cache_view(r->wf_first.get(), npx, r->geo.n_light).n_hits,
+                               sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
     if (dbg_rad) ATX_HIP(hipMemcpyAsync(dbg_rad, d_rad.get(), npx * sizeof(float4), hipMemcpyDeviceToHost, r->stream));
     if (dbg_hit) ATX_HIP(hipMemcpyAsync(dbg_hit, d_hit.get(), npx * dbg_bounces * 4, hipMemcpyDeviceToHost, r->stream));
     if (dbg_shadow) ATX_HIP(hipMemcpyAsync(dbg_shadow, d_sh.get(), npx * dbg_bounces * 4, hipMemcpyDeviceToHost, r->stream));
@@ -237,6 +252,12 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
             path_ms += t;
         }
     }
+    if (read_hits) {
+        r->n_hit = n_hit_read;
+        r->n_hit_exact = true;
+    }
+    for (WfScene& e : table)
+        if (e.hit_px) e.n_hit = r->n_hit;
     r->stats.samples = (uint64_t)npx * frames;
     const uint64_t skipped = wf ? skipped_primaries(table.data(), table.size()) : 0;
     r->stats.rays = ctr[kCtrRays] + ctr[kCtrReused] + skipped;

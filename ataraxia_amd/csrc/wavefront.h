@@ -26,6 +26,40 @@
 
 namespace atx {
 
+// Division of 32-bit unsigned integers by a divisor that stays fixed for
+// many divisions, by a multiply and two shifts (Granlund & Montgomery,
+// "Division by Invariant Integers using Multiplication", PLDI 1994, Fig. 4.1):
+// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+//   t = mulhi(m, n),  floor(n / d) = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0)
+// for all 0 <= n < 2^32 and 1 <= d < 2^32.  The path kernel divides by the
+// scene's light count and job-group length once or more per path; the
+// compiler's general sequence costs about 20 instructions, four of them
+// quarter-rate multiplies.
+struct UDiv {
+    uint32_t d, m, s1, s2;
+};
+__host__ __device__ inline UDiv udiv_make(uint32_t d)
+{
+    uint32_t l = 0;
+    while (l < 32u && (1ull << l) < d)
+        ++l;
+    UDiv r;
+    r.d = d;
+    r.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1u);
+    r.s1 = l < 1u ? l : 1u;
+    r.s2 = l > 1u ? l - 1u : 0u;
+    return r;
+}
+__host__ __device__ __forceinline__ uint32_t udiv(uint32_t n, const UDiv& v)
+{
+    const uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+__host__ __device__ __forceinline__ uint32_t umod(uint32_t n, const UDiv& v)
+{
+    return n - udiv(n, v) * v.d;
+}
+
 struct WfScene {
     // geometry (device layout of device_types.h)
     const float4* sph;
@@ -60,14 +94,20 @@ struct WfScene {
     const float4* dirs;    // primary ray direction per pixel (x, y, z, 0); nullptr: computed per path
     const uint4* first;        // first-bounce records, 4 per pixel (launch_first_hits); nullptr: traced per path
     const float4* first_light; // first-bounce light terms, [light][pixel] (nullptr: none)
-    // With a first-bounce cache, the pixels whose primary ray hits a sphere
-    // (n_hit of them): the launch's jobs cover only these, and k_accumulate
-    // adds the others' sample, the sky colour (or black) of every frame,
-    // without a path.  nullptr: every pixel is a job.  Not with the debug
-    // capture or max_bounces <= 0.
+    // With a first-bounce cache, the pixels whose primary ray hits a sphere:
+    // the launch's jobs cover only these, and k_accumulate adds the others'
+    // sample, the sky colour (or black) of every frame, without a path.
+    // nullptr: every pixel is a job.  Not with the debug capture or
+    // max_bounces <= 0.  The kernels read the list's length from the cache
+    // (job_slots), where k_first_hit leaves it, so a launch can follow
+    // k_first_hit on the stream without the host reading the count first;
+    // n_hit is the length the host lays the launch's chunks out for, at
+    // least the list's (empty chunks past the list end the hand-out).
     const uint32_t* hit_px;
     uint32_t n_hit;
     BvhDev bvh;            // n_nodes = 0: flat loop over all spheres
+    UDiv div_group;        // by kJobGroupPx * frames (set by layout_launch)
+    UDiv div_light;        // by max(n_light, 1) (set by layout_launch)
 };
 
 struct WfPass {
@@ -109,6 +149,18 @@ __host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pi
     const uint32_t left = pixels - kJobGroupPx * g;
     const uint32_t m = left < kJobGroupPx ? left : kJobGroupPx;
     frame = w / m;
+    px = kJobGroupPx * g + (w - frame * m);
+}
+// The same with div_group = udiv_make(kJobGroupPx * frames): one multiply
+// for the group, a shift for the frame but in a scene's last group.
+__host__ __device__ __forceinline__ void job_pixel_frame(uint32_t j, uint32_t pixels, const UDiv& div_group,
+                                                         uint32_t& px, uint32_t& frame)
+{
+    const uint32_t g = udiv(j, div_group);
+    const uint32_t w = j - g * div_group.d;
+    const uint32_t left = pixels - kJobGroupPx * g;
+    const uint32_t m = left < kJobGroupPx ? left : kJobGroupPx;
+    frame = m == kJobGroupPx ? w / kJobGroupPx : w / m;
     px = kJobGroupPx * g + (w - frame * m);
 }
 
@@ -175,13 +227,24 @@ struct FirstCacheView {
     uint32_t* hits;
     uint32_t* n_hits;
 };
+// Bytes of the hit list of a cache of `pixels` pixels (16-byte multiple);
+// the list's length follows it.
+__host__ __device__ inline size_t first_hits_bytes(size_t pixels)
+{
+    return (pixels * 4 + 15) & ~size_t(15);
+}
+// Jobs per frame of a table entry: the length of its hit list, or its pixels.
+__host__ __device__ inline uint32_t job_slots(const uint32_t* hit_px, uint32_t pixels)
+{
+    return hit_px ? *(const uint32_t*)((const char*)hit_px + first_hits_bytes(pixels)) : pixels;
+}
 inline size_t first_light_bytes(size_t pixels, uint32_t lights)
 {
     return lights >= 1 && lights <= kFirstMaxLights ? pixels * lights * 16 : 0;
 }
 inline size_t first_cache_bytes(size_t pixels, uint32_t lights)
 {
-    return pixels * 64 + first_light_bytes(pixels, lights) + ((pixels * 4 + 15) & ~size_t(15)) + 16;
+    return pixels * 64 + first_light_bytes(pixels, lights) + first_hits_bytes(pixels) + 16;
 }
 inline FirstCacheView first_cache_view(void* buf, size_t pixels, uint32_t lights)
 {
@@ -190,7 +253,7 @@ inline FirstCacheView first_cache_view(void* buf, size_t pixels, uint32_t lights
     v.rec = (uint4*)b;
     v.light = first_light_bytes(pixels, lights) ? (float4*)(b + pixels * 64) : nullptr;
     v.hits = (uint32_t*)(b + pixels * 64 + first_light_bytes(pixels, lights));
-    v.n_hits = (uint32_t*)((char*)v.hits + ((pixels * 4 + 15) & ~size_t(15)));
+    v.n_hits = (uint32_t*)((char*)v.hits + first_hits_bytes(pixels));
     return v;
 }
 // Points s.first / s.first_light into buf (first_cache_bytes of s's pixels and lights).

@@ -119,6 +119,8 @@ inline bool layout_launch(WfScene* tab, size_t n, uint32_t chunk_px, float3* sam
         w.n_chunks = (uint32_t)nch;
         w.px_begin = (uint32_t)px;
         w.samples = samples + off;
+        w.div_group = udiv_make(kJobGroupPx * w.frames);
+        w.div_light = udiv_make(w.n_light ? w.n_light : 1u);
         chunks += nch;
         px += npx;
         off += (size_t)(npx * w.frames);

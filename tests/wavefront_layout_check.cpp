@@ -96,6 +96,50 @@ int main()
         }
         CHECK(ok);
     }
+    // division by invariant divisors (multiply and shifts) against / and %
+    {
+        std::vector<uint32_t> ds;
+        for (uint32_t d = 1; d <= 300; ++d) ds.push_back(d);
+        for (uint32_t f = 1; f <= 4096; f = f * 3 / 2 + 1) ds.push_back(8 * f);
+        const uint32_t big[] = {1023u, 1024u, 1025u, 65535u, 65536u, 65537u, 0x7fffffffu, 0x80000000u,
+                                0x80000001u, 0xaaaaaaabu, 0xfffffffeu, 0xffffffffu};
+        for (uint32_t d : big) ds.push_back(d);
+        uint64_t x = 0x9e3779b97f4a7c15ull;
+        for (int k = 0; k < 200; ++k) {
+            x = x * 6364136223846793005ull + 1442695040888963407ull;
+            ds.push_back((uint32_t)(x >> 33) | 1u);
+            ds.push_back((uint32_t)(x >> (40 + k % 20)) + 1u);
+        }
+        bool ok = true;
+        for (uint32_t d : ds) {
+            const UDiv v = udiv_make(d);
+            std::vector<uint32_t> ns = {0u, 1u, d - 1u, d, d + 1u, 2u * d - 1u, 2u * d, 0x7fffffffu,
+                                        0x80000000u, 0xfffffffeu, 0xffffffffu, 0xffffffffu - d};
+            const uint64_t q = 0xffffffffull / d;
+            ns.push_back((uint32_t)(q * d));
+            ns.push_back((uint32_t)(q * d - 1u));
+            for (int k = 0; k < 2000; ++k) {
+                x = x * 6364136223846793005ull + 1442695040888963407ull;
+                ns.push_back((uint32_t)(x >> 32));
+                ns.push_back((uint32_t)(x >> (44 + k % 16)));
+            }
+            for (uint32_t n : ns)
+                if (udiv(n, v) != n / d || umod(n, v) != n % d) ok = false;
+        }
+        CHECK(ok);
+        // the job mapping through the divider equals the plain one
+        for (const auto& sz : sizes) {
+            const uint32_t npx = sz[0], frames = sz[1];
+            const UDiv g = udiv_make(kJobGroupPx * frames);
+            for (uint32_t j = 0; j < npx * frames && ok; ++j) {
+                uint32_t p0, f0, p1, f1;
+                job_pixel_frame(j, npx, frames, p0, f0);
+                job_pixel_frame(j, npx, g, p1, f1);
+                ok = p0 == p1 && f0 == f1;
+            }
+        }
+        CHECK(ok);
+    }
     // a block of 64 consecutive jobs is 768 contiguous bytes of samples
     CHECK(64 * kSampleBytes % 128 == 0 && kMaxChunkPx % 64 == 0);
     for (uint32_t j = 0; j < 64; ++j) {
