@@ -49,6 +49,10 @@ struct Renderer {
     uint32_t wf_frames = kRendererFramesPerLaunch;
     uint32_t wf_waves = 0;   // path-kernel instance (resident waves per SIMD); 0: default
     DevBuf accum, comp, image, counters;
+    // pinned host memory the call's statistics are copied into (counters,
+    // then the hit list's length): copies into pageable memory block the
+    // host until the stream is idle, one after the other
+    unsigned long long* readback = nullptr;
     uint32_t width = 0, height = 0;
     atx_settings settings{1, 0, {0, 0}, 15};  // Scene.h:49-56 defaults
     uint32_t frame_index = 1;
@@ -230,12 +234,13 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         }
     }
     ATX_HIP(hipEventRecord(r->ev1, r->stream));
-    unsigned long long ctr[kCtrCount];
-    ATX_HIP(hipMemcpyAsync(ctr, r->counters.get(), sizeof(ctr), hipMemcpyDeviceToHost, r->stream));
+    unsigned long long* ctr = r->readback;
+    ATX_HIP(hipMemcpyAsync(ctr, r->counters.get(), kCtrCount * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           r->stream));
     const bool read_hits = use_first && !r->n_hit_exact;
-    uint32_t n_hit_read = 0;
+    uint32_t* n_hit_read = reinterpret_cast<uint32_t*>(r->readback + kCtrCount);
     if (read_hits)
-        ATX_HIP(hipMemcpyAsync(&n_hit_read, first_cache_view(r->wf_first.get(), npx, r->geo.n_light).n_hits,
+        ATX_HIP(hipMemcpyAsync(n_hit_read, first_cache_view(r->wf_first.get(), npx, r->geo.n_light).n_hits,
                                sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
     if (dbg_rad) ATX_HIP(hipMemcpyAsync(dbg_rad, d_rad.get(), npx * sizeof(float4), hipMemcpyDeviceToHost, r->stream));
     if (dbg_hit) ATX_HIP(hipMemcpyAsync(dbg_hit, d_hit.get(), npx * dbg_bounces * 4, hipMemcpyDeviceToHost, r->stream));
@@ -253,7 +258,7 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         }
     }
     if (read_hits) {
-        r->n_hit = n_hit_read;
+        r->n_hit = *n_hit_read;
         r->n_hit_exact = true;
     }
     for (WfScene& e : table)
@@ -292,6 +297,8 @@ int atx_renderer_create(atx_renderer** out)
     if (e == hipSuccess) e = hipEventCreate(&r->ev0);
     if (e == hipSuccess) e = hipEventCreate(&r->ev1);
     if (e == hipSuccess) e = r->counters.reserve(kCtrCount * sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&r->readback, (kCtrCount + 1) * sizeof(unsigned long long), hipHostMallocDefault);
     if (e != hipSuccess) {
         set_error("renderer create: %s", hipGetErrorString(e));
         atx_renderer_destroy(reinterpret_cast<atx_renderer*>(r));
@@ -324,6 +331,7 @@ int atx_renderer_destroy(atx_renderer* h)
     if (r->ev1) (void)hipEventDestroy(r->ev1);
     for (hipEvent_t ev : r->wf_ev) (void)hipEventDestroy(ev);
     if (r->stream) (void)hipStreamDestroy(r->stream);
+    if (r->readback) (void)hipHostFree(r->readback);
     delete r;
     return ATX_OK;
 }
