@@ -57,6 +57,21 @@ __device__ __forceinline__ float bvh_inv_dir(float v)
     return __builtin_amdgcn_rcpf(c);
 }
 
+// min(a, b) as one v_min_f32.  fminf on a value carried around the node loop
+// (tlim) costs a canonicalising v_max_f32 per node besides the min; neither
+// operand can be a signalling NaN here (both come out of arithmetic), for
+// which that canonicalisation exists.
+__device__ __forceinline__ float bvh_min_raw(float a, float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+#else
+    return __builtin_fminf(a, b);
+#endif
+}
+
 // Radius excess sqrt(r^2 + q) - r, q = 2^-17 (D^2 + r^2), in a form without
 // cancellation; hardware sqrt and reciprocal (a few ulp, which the caller's
 // 2^-16 widening covers).
@@ -129,7 +144,7 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
         const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(ax, bx), __builtin_fminf(ay, by)),
                                          __builtin_fmaxf(__builtin_fminf(az, bz), 0.0f));
         const float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
-                                         __builtin_fminf(__builtin_fmaxf(az, bz), tlim));
+                                         bvh_min_raw(__builtin_fmaxf(az, bz), tlim));
         uint32_t next = w.w & 0xffffu;
         if (tn <= tf) {
             const uint32_t leaf = w.w >> 16;
@@ -139,7 +154,7 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
                 const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
                 for (uint32_t k = 0; k < cnt; ++k) {
                     if (filter_sphere(lsph[first + k], lidx[first + k], o, d, a, f))
-                        tlim = __builtin_fminf(tlim, f.best_hi * inv_a_up);
+                        tlim = bvh_min_raw(tlim, f.best_hi * inv_a_up);
                 }
                 q.tests += cnt;
                 // t of the best sphere <= tlim, and tlim*tlim < dist2 proves it occludes
@@ -151,7 +166,7 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
     }
 
     Hit h;
-    if (f.amb) {
+    if (f.amb != 0u) {
         ++q.exact;
         q.tests += n_all;
         h = trace_exact(flat, n_all, o, d);

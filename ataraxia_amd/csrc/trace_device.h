@@ -85,7 +85,8 @@ struct Filter {
     float best_lo;   // interval of the best numerator so far
     float best_hi;
     int32_t best;    // its sphere index, -1 = none
-    bool amb;        // the filter could not decide: re-run the exact loop
+    uint32_t amb;    // nonzero: the filter could not decide, re-run the exact loop (an
+                     // integer, so that loops carry it in a VGPR, not a lane mask)
 };
 
 __device__ __forceinline__ Filter filter_init()
@@ -94,7 +95,7 @@ __device__ __forceinline__ Filter filter_init()
     f.best_lo = __builtin_inff();
     f.best_hi = __builtin_inff();
     f.best = -1;
-    f.amb = false;
+    f.amb = 0u;
     return f;
 }
 
@@ -124,7 +125,7 @@ __device__ __forceinline__ bool filter_sphere(float4 s, int32_t id, f3 o, f3 d, 
     const float lo = __builtin_fmaf(u, -kMargin, nn);
     const bool better = (hi < f.best_lo) & (lo > 0.0f);
     const bool near = (hi > 0.0f) & (lo < f.best_hi);
-    f.amb |= near & !better;
+    f.amb |= (near & !better) ? 1u : 0u;
     f.best_lo = better ? lo : f.best_lo;
     f.best_hi = better ? hi : f.best_hi;
     f.best = better ? id : f.best;
@@ -138,7 +139,7 @@ __device__ __forceinline__ Hit trace_filtered(const Src& src, uint32_t n, f3 o, 
     Filter f = filter_init();
     for (uint32_t i = 0; i < n; ++i)
         filter_sphere(src(i), (int32_t)i, o, d, a, f);
-    const bool amb = f.amb;
+    const bool amb = f.amb != 0u;
     const int32_t best = f.best;
     Hit h;
     if (amb) {
