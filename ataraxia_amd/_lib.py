@@ -144,6 +144,7 @@ PROTOTYPES = {
     "atx_renderer_render_debug": (_I, [_P, C.POINTER(CameraDesc), _P, _P, _P, _U32]),
     "atx_renderer_set_kernel": (_I, [_P, _I]),
     "atx_renderer_set_accel": (_I, [_P, _I]),
+    "atx_renderer_reserve": (_I, [_P, _U32]),
     "atx_render_frames_noalloc": (_I, [_P, _U64, _P, _U64, _P, _U64, C.POINTER(Settings), C.POINTER(CameraDesc),
                                        _U32, _U32, _I, _I, _P, _P, _P, _P]),
     "atx_batch_create": (_I, [C.POINTER(_P)]),

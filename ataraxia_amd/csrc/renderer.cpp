@@ -404,6 +404,29 @@ int atx_renderer_get_settings(const atx_renderer* h, atx_settings* s)
     return ATX_OK;
 }
 
+int atx_renderer_reserve(atx_renderer* h, uint32_t frames)
+{
+    Renderer* r = reinterpret_cast<Renderer*>(h);
+    ATX_CHECK_ARG(r, "renderer is NULL");
+    ATX_CHECK_ARG(r->width > 0 && r->height > 0, "reserve before on_resize");
+    if (frames == 0) return ATX_OK;
+    const size_t npx = (size_t)r->width * r->height;
+    // the largest launch render() forms for such a call (as it sizes them)
+    const uint32_t fit = (uint32_t)std::max<size_t>(1, kSampleBufferBytes / (npx * kSampleBytes));
+    const uint32_t per_launch = std::min(std::min(frames, r->wf_frames), fit);
+    ATX_HIP(r->wf_samples.reserve(npx * per_launch * kSampleBytes));
+    // growing a buffer drops its contents: the per-camera caches are redone
+    if (r->wf_dirs.bytes() < npx * sizeof(float4)) {
+        ATX_HIP(r->wf_dirs.reserve(npx * sizeof(float4)));
+        r->dirs_valid = r->first_valid = false;
+    }
+    if (r->has_scene && r->wf_first.bytes() < first_cache_bytes(npx, r->geo.n_light)) {
+        ATX_HIP(r->wf_first.reserve(first_cache_bytes(npx, r->geo.n_light)));
+        r->first_valid = false;
+    }
+    return ATX_OK;
+}
+
 int atx_renderer_reset_frame_index(atx_renderer* h)
 {
     Renderer* r = reinterpret_cast<Renderer*>(h);

@@ -96,6 +96,11 @@ class Renderer:
             self.set_scene(scene)
         check(lib.atx_renderer_render(self._h, C.byref(camera.to_c())))
 
+    def reserve(self, frames: int) -> None:
+        """Allocates the device buffers that render calls of up to `frames`
+        frames at the current size need, so that they allocate nothing."""
+        check(lib.atx_renderer_reserve(self._h, frames))
+
     def render_frames(self, camera: Camera, frames: int) -> None:
         check(lib.atx_renderer_render_frames(self._h, C.byref(camera.to_c()), frames))
 

@@ -158,10 +158,11 @@ def cpu_baseline(sc, target_s):
 
 
 def time_single_scene(sc, kernel, accel, steps, warmup):
-    """Times `steps` frames of one render call.  The warm-up frames use a
-    camera moved by 1e-3, so that the renderer's buffers exist but its
-    per-camera work (primary directions, first-bounce cache) is redone
-    inside the timed region, as a fresh camera would need it."""
+    """Times `steps` frames of one render call.  The renderer reserves its
+    buffers for calls of `steps` frames and the warm-up frames use a camera
+    moved by 1e-3, so that the renderer's buffers exist but its per-camera
+    work (primary directions, first-bounce cache) is redone inside the timed
+    region, as a fresh camera would need it."""
     import dataclasses
     from ataraxia_amd import Renderer
     from ataraxia_amd._lib import lib
@@ -173,6 +174,7 @@ def time_single_scene(sc, kernel, accel, steps, warmup):
     r.on_resize(cam.width, cam.height)
     r.set_settings(sc.settings)
     r.set_scene(sc)
+    r.reserve(steps)
     if warmup:
         r.render_frames(warm_cam, warmup)
     r.reset_frame_index()

@@ -176,6 +176,13 @@ int atx_renderer_render_debug(atx_renderer* r, const atx_camera* cam, float* rad
 int atx_renderer_set_kernel(atx_renderer* r, int kind);
 /* ATX_ACCEL_BVH (default) or ATX_ACCEL_NONE; results are identical.        */
 int atx_renderer_set_accel(atx_renderer* r, int accel);
+/* Allocates now the device buffers that render calls of up to `frames`
+ * frames at the current size need (the persistent kernel's sample buffer,
+ * primary directions, first-bounce cache once a scene is set), so that such
+ * calls allocate nothing; a later call that needs more still grows them.
+ * No counterpart in the reference (Renderer.cu:173-249 allocates and frees
+ * per frame); for callers that time or pace render calls.                  */
+int atx_renderer_reserve(atx_renderer* r, uint32_t frames);
 
 /* ---- Allocation-free rendering ---------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -82,6 +82,7 @@ def test_argument_errors_return_codes():
     assert rc == _lib.ATX_ERR_INVALID and "need 48" in _lib.last_error()
     assert _lib.lib.atx_renderer_create(None) == _lib.ATX_ERR_INVALID
     assert _lib.lib.atx_renderer_destroy(None) == _lib.ATX_OK
+    assert _lib.lib.atx_renderer_reserve(None, 4) == _lib.ATX_ERR_INVALID
 
 
 @pytest.mark.skipif(_lib.device_count() > 0, reason="only meaningful without a device")

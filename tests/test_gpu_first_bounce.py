@@ -202,3 +202,31 @@ def test_cache_kept_across_accel_and_settings_changes():
         out[kernel] = accs
     for k, (x, y) in enumerate(zip(out[ATX_KERNEL_PLAIN], out[ATX_KERNEL_WAVEFRONT])):
         assert np.array_equal(x, y), k
+
+
+def test_reserve_keeps_results_and_redoes_dropped_caches():
+    """Renderer.reserve: before the first call, between calls of one camera
+    (the sample buffer grows) and after a resize, the accumulation equals a
+    renderer that never reserved; growing the buffers drops the per-camera
+    caches, which the next call rebuilds."""
+    sc = CASES["plain-cpu"]
+    cam = sc.camera
+    ref, _, _, _ = _render(sc, ATX_KERNEL_WAVEFRONT, 6, 2)
+    with Renderer() as r:
+        r.on_resize(cam.width, cam.height)
+        r.set_settings(sc.settings)
+        r.reserve(4)                 # no scene yet: sample buffer and primary directions only
+        r.set_scene(sc)
+        r.render_frames(cam, 2)
+        r.reserve(64)                # more frames per call than reserved before
+        r.render_frames(cam, 2)
+        r.render_frames(cam, 2)
+        assert np.array_equal(r.get_accumulation(), ref)
+        small = scene_from_camera_overrides(sc, cam.width // 2, cam.height // 2)
+        r.on_resize(small.camera.width, small.camera.height)
+        r.reserve(2)
+        r.reset_frame_index()
+        r.render_frames(small.camera, 2)
+        acc_small = r.get_accumulation().copy()
+    ref_small, _, _, _ = _render(small, ATX_KERNEL_WAVEFRONT, 2, 2)
+    assert np.array_equal(acc_small, ref_small)
