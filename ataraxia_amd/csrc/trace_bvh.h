@@ -44,6 +44,12 @@ namespace atx {
 #define ATX_GLOBAL
 #endif
 
+// Spheres found in the tree's leaves are named by their leaf slot (tagged)
+// while the query runs and mapped to their scene index once at its end: the
+// filter's decisions do not depend on sphere indices (ties go to the exact
+// loop), and the index load then stays off the leaf loop's critical path.
+constexpr int32_t kLeafSlot = 0x40000000;
+
 struct BvhQuery {
     uint32_t tests = 0;   // sphere tests
     uint32_t boxes = 0;   // node box tests
@@ -153,7 +159,7 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
             } else {
                 const uint32_t first = (leaf >> 2) - 1u, cnt = (leaf & 3u) + 1u;
                 for (uint32_t k = 0; k < cnt; ++k) {
-                    if (filter_sphere(lsph[first + k], lidx[first + k], o, d, a, f))
+                    if (filter_sphere(lsph[first + k], kLeafSlot | (int32_t)(first + k), o, d, a, f))
                         tlim = bvh_min_raw(tlim, f.best_hi * inv_a_up);
                 }
                 q.tests += cnt;
@@ -171,8 +177,9 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
         q.tests += n_all;
         h = trace_exact(flat, n_all, o, d);
     } else {
-        h.id = f.best;
-        h.t = f.best >= 0 ? exact_t(flat((uint32_t)f.best), o, d, a) : FLT_MAX;
+        const int32_t best = f.best >= kLeafSlot ? lidx[f.best - kLeafSlot] : f.best;
+        h.id = best;
+        h.t = best >= 0 ? exact_t(flat((uint32_t)best), o, d, a) : FLT_MAX;
     }
     return h;
 }
