@@ -227,6 +227,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const uint32_t P = a.chunk_px;
+    // The instances for 7 and 8 resident waves per SIMD park the path state
+    // that a trace does not read (18 values per lane) in LDS while the trace
+    // runs: the state then needs no VGPRs there, and the trace fits the
+    // smaller register budget without spilling (18 KB of LDS per
+    // 256-lane workgroup, 147 KB per CU at 8 waves per SIMD).
+    constexpr bool kPark = kWaves >= 7;
+    constexpr uint32_t kParked = 18;
+    __shared__ uint32_t parked_all[4][kPark ? kParked : 1][64];
+    uint32_t(*parked)[64] = parked_all[threadIdx.x >> 6];
 
     // ---- wave-uniform scheduling state ----
     CScene* scenes = as_const(a.scenes);
@@ -426,6 +435,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
             const f3 d = shadow ? L : rd;
             const SphSrc flat{as_const(S->sph)};
             Hit h;
+            if constexpr (kPark) {
+                const uint32_t v[kParked] = {__float_as_uint(col.x), __float_as_uint(col.y), __float_as_uint(col.z),
+                                             __float_as_uint(tp.x),  __float_as_uint(tp.y),  __float_as_uint(tp.z),
+                                             __float_as_uint(nrm.x), __float_as_uint(nrm.y), __float_as_uint(nrm.z),
+                                             seed, (uint32_t)bounce, px, job, (uint32_t)mid, lidx, reused, tests,
+                                             boxes};
+                for (uint32_t k = 0; k < kParked; ++k)
+                    parked[k][lane] = v[k];
+                // keep the compiler from forwarding the stored values past the trace
+                __asm__ volatile("" ::: "memory");
+            }
+            uint32_t dt = 0, db = 0;   // the trace's sphere and box tests
             if (S->bvh.n_nodes) {
                 // the debug capture records the closest occluder; otherwise any proven one will do
                 const BvhDev B = S->bvh;
@@ -434,15 +455,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 h = trace_bvh(B, flat, S->n_sph, o, d, qd2, q);
                 if (q.exact)
                     atomicAdd(a.counters + kCtrExact, (unsigned long long)q.exact);
-                tests += q.tests;
-                boxes += q.boxes;
+                dt = q.tests;
+                db = q.boxes;
             } else {
                 uint32_t ex = 0;
                 h = trace_filtered(flat, S->n_sph, o, d, ex);
                 if (ex)
                     atomicAdd(a.counters + kCtrExact, (unsigned long long)ex);
-                tests += S->n_sph;
+                dt = S->n_sph;
             }
+            if constexpr (kPark) {
+                __asm__ volatile("" ::: "memory");
+                col = mk(__uint_as_float(parked[0][lane]), __uint_as_float(parked[1][lane]),
+                         __uint_as_float(parked[2][lane]));
+                tp = mk(__uint_as_float(parked[3][lane]), __uint_as_float(parked[4][lane]),
+                        __uint_as_float(parked[5][lane]));
+                nrm = mk(__uint_as_float(parked[6][lane]), __uint_as_float(parked[7][lane]),
+                         __uint_as_float(parked[8][lane]));
+                seed = parked[9][lane];
+                bounce = (int32_t)parked[10][lane];
+                px = parked[11][lane];
+                job = parked[12][lane];
+                mid = (int32_t)parked[13][lane];
+                lidx = parked[14][lane];
+                reused = parked[15][lane];
+                tests = parked[16][lane];
+                boxes = parked[17][lane];
+            }
+            tests += dt;
+            boxes += db;
 
             // ---------------- shading (divergent) ----------------
             if (!shadow) {

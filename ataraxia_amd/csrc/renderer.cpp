@@ -148,7 +148,11 @@ static int renderer_launch_frames(Renderer* r, const atx_camera* cam, uint32_t f
         for (const WfScene& e : table) most = std::max(most, launch_samples(&e, 1));
         ATX_HIP(r->wf_samples.reserve(most * kSampleBytes));
         shape.flat_only = wf_flat_only(table.data(), table.size());
-        shape.waves = r->wf_waves;
+        // the instance: the caller's, else by the jobs of the largest launch
+        // (every pixel in every frame, an upper bound of the hit list's)
+        shape.waves = r->wf_waves ? r->wf_waves
+                      : shape.flat_only ? 0u
+                                        : wf_tree_waves((uint64_t)npx * launch_frames[0]);
         shape.workgroups = wavefront_full_grid(shape.flat_only, shape.waves);
         // The first-bounce cache pays from two frames per camera on: a call
         // of one frame uses it once the camera has stayed the same since the

@@ -101,7 +101,8 @@ struct Server {
     DevBuf samples;                 // the tick's sample buffer (shared by the active requests)
     size_t samples_excess = 0;      // its capacity beyond the active requests' shares, charged to hbm_in_use
     double rendered = 0.0, busy_s = 0.0;   // samples rendered and GPU seconds so far (worker)
-    uint32_t grid = 0, grid_flat = 0;   // persistent grids: ticks with trees, flat-loop-only ticks
+    // persistent grids: ticks with trees (6- and 8-wave instances), flat-loop-only ticks
+    uint32_t grid = 0, grid8 = 0, grid_flat = 0;
     int device = 0;
 };
 
@@ -381,7 +382,8 @@ static bool launch_tick(Server* s, std::vector<WfScene>& table, std::vector<std:
         s->samples_excess = excess;
     }
     const bool flat = wf_flat_only(table.data(), table.size());
-    const WfLaunch shape{flat ? s->grid_flat : s->grid, flat};
+    const uint32_t waves = flat ? 0u : wf_tree_waves(launch_jobs(table.data(), table.size()));
+    const WfLaunch shape{flat ? s->grid_flat : waves == 8u ? s->grid8 : s->grid, flat, waves};
     const uint32_t P =
         s->cfg.chunk_px ? s->cfg.chunk_px : auto_chunk_px(launch_jobs(table.data(), table.size()), shape.workgroups);
     if (ok) ok = layout_launch(table.data(), table.size(), P, s->samples.as<float3>(), &wp);
@@ -586,6 +588,7 @@ int atx_server_create(atx_server** out, const atx_server_config* cfg)
     ATX_HIP(s->counters.reserve(kCtrCount * sizeof(unsigned long long)));
     ATX_HIP(hipMemset(s->counters.get(), 0, kCtrCount * sizeof(unsigned long long)));
     s->grid = wavefront_full_grid(false);
+    s->grid8 = wavefront_full_grid(false, 8);
     s->grid_flat = wavefront_full_grid(true);
     s->epoch = std::chrono::steady_clock::now();
     Server* raw = s.release();

@@ -181,6 +181,15 @@ constexpr uint32_t kWfWavesFlat = 7;
 // Register budgets the path kernel is compiled for (resident waves per
 // SIMD); a launch picks one (the batch autotuner tries them all).
 constexpr uint32_t kWfWavesMin = 5, kWfWavesMax = 8;
+// Launches with trees of at least this many jobs take the 8-wave instance,
+// which parks the path state in LDS during traces (1080p workload, path
+// kernel: 64 / 128 / 256 frames 4.5 / 4.7 / 4.7 % faster than 6 waves; 20
+// frames, 27 M jobs, 1.9 % slower).
+constexpr uint64_t kWfWaves8MinJobs = uint64_t(64) << 20;
+__host__ __device__ __forceinline__ uint32_t wf_tree_waves(uint64_t jobs)
+{
+    return jobs >= kWfWaves8MinJobs ? 8u : kWfWavesTree;
+}
 // The instance a launch uses: `waves` when it names a compiled instance,
 // else the default for its scenes.
 __host__ __device__ __forceinline__ uint32_t wf_waves(uint32_t waves, bool flat_only)
