@@ -48,6 +48,8 @@ namespace atx {
 // while the query runs and mapped to their scene index once at its end: the
 // filter's decisions do not depend on sphere indices (ties go to the exact
 // loop), and the index load then stays off the leaf loop's critical path.
+// A scene with a tree has at most kBvhMaxTree + kBvhMaxBig (16398) spheres,
+// so scene indices never carry the tag.
 constexpr int32_t kLeafSlot = 0x40000000;
 
 struct BvhQuery {
