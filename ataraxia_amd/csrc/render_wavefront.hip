@@ -93,20 +93,18 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-// Adds the wave's per-lane counts to the launch's counters and clears them
-// (every lane of the wave calls it).
+// Adds the wave's per-lane test counts to the launch's counters and clears
+// them (every lane of the wave calls it).
 __device__ __forceinline__ void flush_counters(unsigned long long* ctr, uint32_t lane, uint32_t& tests,
-                                               uint32_t& boxes, uint32_t& reused)
+                                               uint32_t& boxes)
 {
     const unsigned long long ts = wave_sum(tests);
     const unsigned long long bx = wave_sum(boxes);
-    const unsigned long long ru = wave_sum(reused);
     if (lane == 0) {
         if (ts) atomicAdd(ctr + kCtrTests, ts);
         if (bx) atomicAdd(ctr + kCtrBoxes, bx);
-        if (ru) atomicAdd(ctr + kCtrReused, ru);
     }
-    tests = boxes = reused = 0u;
+    tests = boxes = 0u;
 }
 
 // ---- one bounce of the per-pixel path (Renderer.cu:319-381), in pieces
@@ -208,16 +206,25 @@ __device__ __forceinline__ bool next_bounce(CScene* S, int32_t mid, f3 nrm, f3 s
     return false;
 }
 
-// The sample of the scene's job `job` (pixel px; `slots` jobs per frame),
-// and the last frame's debug capture.
-__device__ __forceinline__ void store_sample(CScene* S, uint32_t slots, uint32_t job, uint32_t px, f3 col)
+// The pixel of the scene's job `job` (`slots` jobs per frame): the path
+// state does not carry it, only the debug captures need it.
+__device__ __forceinline__ uint32_t job_pixel(CScene* S, uint32_t slots, uint32_t job)
+{
+    uint32_t slot, f;
+    job_pixel_frame(job, slots, S->div_group, slot, f);
+    return S->hit_px ? S->hit_px[slot] : slot;
+}
+
+// The sample of the scene's job `job` (`slots` jobs per frame), and the
+// last frame's debug capture.
+__device__ __forceinline__ void store_sample(CScene* S, uint32_t slots, uint32_t job, f3 col)
 {
     S->samples[job] = make_float3(col.x, col.y, col.z);
     if (S->dbg_radiance) {
         uint32_t slot, f;
-        job_pixel_frame(job, slots, S->frames, slot, f);
+        job_pixel_frame(job, slots, S->div_group, slot, f);
         if (f == S->frames - 1u)
-            S->dbg_radiance[px] = make_float4(col.x, col.y, col.z, 1.0f);
+            S->dbg_radiance[S->hit_px ? S->hit_px[slot] : slot] = make_float4(col.x, col.y, col.z, 1.0f);
     }
 }
 
@@ -228,12 +235,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
     const unsigned long long lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const uint32_t P = a.chunk_px;
     // The instances for 7 and 8 resident waves per SIMD park the path state
-    // that a trace does not read (18 values per lane) in LDS while the trace
+    // that a trace does not read (17 values per lane) in LDS while the trace
     // runs: the state then needs no VGPRs there, and the trace fits the
-    // smaller register budget without spilling (18 KB of LDS per
-    // 256-lane workgroup, 147 KB per CU at 8 waves per SIMD).
+    // smaller register budget without spilling (17 KB of LDS per
+    // 256-lane workgroup, 136 KB per CU at 8 waves per SIMD).
     constexpr bool kPark = kWaves >= 7;
-    constexpr uint32_t kParked = 18;
+    constexpr uint32_t kParked = 17;
     __shared__ uint32_t parked_all[4][kPark ? kParked : 1][64];
     uint32_t(*parked)[64] = parked_all[threadIdx.x >> 6];
 
@@ -251,19 +258,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
 
     // ---- per-lane path state ----
     uint32_t state = kIdle;
-    uint32_t px = 0, job = 0, seed = 0;   // pixel, the scene's job index, PCG state
+    uint32_t job = 0, seed = 0;   // the scene's job index, PCG state
     int32_t bounce = 0;
-    f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, tp = ro, col = ro;
-    f3 so = ro, nrm = ro, L = ro;   // shadow/next origin, normal, light direction
-    float dist2 = 0.0f;
+    // (ro, rd): the lane's next ray -- the primary or continuation ray in
+    // kCont, the shadow ray (from the hit towards the light) in kShadow;
+    // vd: in kShadow, the direction of the ray that made the hit (the view
+    // direction of the light's Cook-Torrance term)
+    f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, vd = ro, tp = ro, col = ro;
+    f3 nrm = ro;                    // normal at the current hit
+    float dist2 = 0.0f;             // in kShadow: squared distance to the light
     int32_t mid = 0;                // material of the current hit
     uint32_t lidx = 0;              // light of the current shadow ray
     bool capture = false;           // this path is the last frame's: write the debug capture
-    // statistics: rays per wave; test counts per lane, added to the launch's
-    // 64-bit counters at the end (and earlier if one could overflow); exact
-    // retraces (rare) straight to their counter
-    unsigned long long rays = 0;
-    uint32_t tests = 0, boxes = 0, reused = 0;
+    // statistics: rays and cached rays per wave; test counts per lane, added
+    // to the launch's 64-bit counters at the end (and earlier if one could
+    // overflow); exact retraces (rare) straight to their counter
+    unsigned long long rays = 0, reused = 0;
+    uint32_t tests = 0, boxes = 0;
 
     for (;;) {
         // ---------------- hand jobs to idle lanes (uniform control flow) ----------------
@@ -284,9 +295,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const uint32_t avail = blk_end - blk_next;
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
                 const uint32_t nidle = (uint32_t)__popcll(idle);
+                uint32_t cached_rays = 0;   // this lane's rays taken from the first-bounce cache
                 if (state == kIdle && rank < avail) {
                     job = blk0 + blk_next + rank;
-                    uint32_t frame;
+                    uint32_t px, frame;
                     job_pixel_frame(job, slots, S->div_group, px, frame);
                     if (S->hit_px)
                         px = S->hit_px[px];
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         const float4 lt = S->first_light ? S->first_light[(size_t)lsel * (S->width * S->height) + px]
                                                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         const int32_t hid = (int32_t)r0.x;
-                        ++reused;
+                        cached_rays = 1;
                         bool done = false;
                         if (hid < 0) {
                             if (S->sky_light)
@@ -348,28 +360,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                             Hit h;
                             h.id = hid;
                             h.t = __uint_as_float(r0.y);
+                            f3 so, L;
                             shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2);
+                            vd = rd;
+                            ro = so;
+                            rd = L;
                             state = kShadow;
                         } else {
                             nrm = mk(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
-                            so = mk(__uint_as_float(r1.w), __uint_as_float(r2.x), __uint_as_float(r2.y));
+                            const f3 so = mk(__uint_as_float(r1.w), __uint_as_float(r2.x), __uint_as_float(r2.y));
                             col = mk(__uint_as_float(r2.z), __uint_as_float(r2.w), __uint_as_float(r3.x));
                             mid = (int32_t)r0.w;
                             if (nl > 0u) {
                                 // the frame's light, and its term if it reaches the hit
                                 lidx = lsel;
-                                ++reused;
+                                cached_rays = 2;
                                 if ((r0.z >> lidx) & 1u)
                                     col = add(col, xyz(lt));
                             }
                             done = next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed);
                         }
                         if (done) {
-                            store_sample(S, slots, job, px, col);
+                            store_sample(S, slots, job, col);
                             state = kIdle;
                         }
                     }
                 }
+                reused += (unsigned long long)(__popcll(__ballot(cached_rays >= 1u)) +
+                                               __popcll(__ballot(cached_rays >= 2u)));
                 blk_next += nidle < avail ? nidle : avail;
                 continue;
             }
@@ -431,16 +449,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
         rays += (unsigned long long)__popcll(__ballot(traces));
         if (traces) {
             const bool shadow = state == kShadow;
-            const f3 o = shadow ? so : ro;
-            const f3 d = shadow ? L : rd;
             const SphSrc flat{as_const(S->sph)};
             Hit h;
             if constexpr (kPark) {
                 const uint32_t v[kParked] = {__float_as_uint(col.x), __float_as_uint(col.y), __float_as_uint(col.z),
                                              __float_as_uint(tp.x),  __float_as_uint(tp.y),  __float_as_uint(tp.z),
                                              __float_as_uint(nrm.x), __float_as_uint(nrm.y), __float_as_uint(nrm.z),
-                                             seed, (uint32_t)bounce, px, job, (uint32_t)mid, lidx, reused, tests,
-                                             boxes};
+                                             __float_as_uint(vd.x),  __float_as_uint(vd.y),  __float_as_uint(vd.z),
+                                             seed, (uint32_t)bounce, job, (uint32_t)mid, lidx};
                 for (uint32_t k = 0; k < kParked; ++k)
                     parked[k][lane] = v[k];
                 // keep the compiler from forwarding the stored values past the trace
@@ -452,14 +468,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const BvhDev B = S->bvh;
                 BvhQuery q;
                 const float qd2 = (shadow && !(capture && S->dbg_shadow)) ? dist2 : -1.0f;
-                h = trace_bvh(B, flat, S->n_sph, o, d, qd2, q);
+                h = trace_bvh(B, flat, S->n_sph, ro, rd, qd2, q);
                 if (q.exact)
                     atomicAdd(a.counters + kCtrExact, (unsigned long long)q.exact);
                 dt = q.tests;
                 db = q.boxes;
             } else {
                 uint32_t ex = 0;
-                h = trace_filtered(flat, S->n_sph, o, d, ex);
+                h = trace_filtered(flat, S->n_sph, ro, rd, ex);
                 if (ex)
                     atomicAdd(a.counters + kCtrExact, (unsigned long long)ex);
                 dt = S->n_sph;
@@ -472,15 +488,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                         __uint_as_float(parked[5][lane]));
                 nrm = mk(__uint_as_float(parked[6][lane]), __uint_as_float(parked[7][lane]),
                          __uint_as_float(parked[8][lane]));
-                seed = parked[9][lane];
-                bounce = (int32_t)parked[10][lane];
-                px = parked[11][lane];
-                job = parked[12][lane];
-                mid = (int32_t)parked[13][lane];
-                lidx = parked[14][lane];
-                reused = parked[15][lane];
-                tests = parked[16][lane];
-                boxes = parked[17][lane];
+                if (shadow)
+                    vd = mk(__uint_as_float(parked[9][lane]), __uint_as_float(parked[10][lane]),
+                            __uint_as_float(parked[11][lane]));
+                seed = parked[12][lane];
+                bounce = (int32_t)parked[13][lane];
+                job = parked[14][lane];
+                mid = (int32_t)parked[15][lane];
+                lidx = parked[16][lane];
             }
             tests += dt;
             boxes += db;
@@ -488,41 +503,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
             // ---------------- shading (divergent) ----------------
             if (!shadow) {
                 if (capture && S->dbg_hit && (uint32_t)bounce < S->dbg_bounces)
-                    S->dbg_hit[(size_t)px * S->dbg_bounces + bounce] = h.id;
+                    S->dbg_hit[(size_t)job_pixel(S, slots, job) * S->dbg_bounces + bounce] = h.id;
                 if (h.id < 0) {
                     if (S->sky_light)
                         col = add(col, mul(mk(0.6f, 0.7f, 0.9f), tp));
                     end_path = true;
-                } else if (shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2)) {
-                    state = kShadow;
                 } else {
-                    end_bounce = true;
+                    f3 so, L;
+                    if (shade_hit(S, h, ro, rd, tp, seed, col, nrm, so, mid, lidx, L, dist2)) {
+                        vd = rd;
+                        rd = L;
+                        state = kShadow;
+                    } else {
+                        end_bounce = true;
+                    }
+                    ro = so;
                 }
             } else {
                 if (capture && S->dbg_shadow && (uint32_t)bounce < S->dbg_bounces)
-                    S->dbg_shadow[(size_t)px * S->dbg_bounces + bounce] = h.id;
+                    S->dbg_shadow[(size_t)job_pixel(S, slots, job) * S->dbg_bounces + bounce] = h.id;
                 if (lit_by(h, dist2))
-                    add_light(S, mid, lidx, nrm, rd, L, tp, col);
+                    add_light(S, mid, lidx, nrm, vd, rd, tp, col);
                 end_bounce = true;
             }
         }
         if (end_bounce) {
-            if (next_bounce(S, mid, nrm, so, tp, ro, rd, bounce, seed))
+            // the next ray leaves from the hit's offset origin, which ro holds
+            if (next_bounce(S, mid, nrm, ro, tp, ro, rd, bounce, seed))
                 end_path = true;
             else
                 state = kCont;
         }
-        if (__any(tests >= 0x80000000u || boxes >= 0x80000000u || reused >= 0x80000000u))
-            flush_counters(a.counters, lane, tests, boxes, reused);
+        if (__any(tests >= 0x80000000u || boxes >= 0x80000000u))
+            flush_counters(a.counters, lane, tests, boxes);
         if (end_path) {
-            store_sample(S, slots, job, px, col);
+            store_sample(S, slots, job, col);
             state = kIdle;
         }
     }
 
-    flush_counters(a.counters, lane, tests, boxes, reused);
+    flush_counters(a.counters, lane, tests, boxes);
     if (lane == 0 && rays)
         atomicAdd(a.counters + kCtrRays, rays);
+    if (lane == 0 && reused)
+        atomicAdd(a.counters + kCtrReused, reused);
 }
 
 // Adds every pixel's samples of the launch to its accumulation in frame order

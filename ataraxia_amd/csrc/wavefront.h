@@ -182,10 +182,11 @@ constexpr uint32_t kWfWavesFlat = 7;
 // SIMD); a launch picks one (the batch autotuner tries them all).
 constexpr uint32_t kWfWavesMin = 5, kWfWavesMax = 8;
 // Launches with trees of at least this many jobs take the 8-wave instance,
-// which parks the path state in LDS during traces (1080p workload, path
-// kernel: 64 / 128 / 256 frames 4.5 / 4.7 / 4.7 % faster than 6 waves; 20
-// frames, 27 M jobs, 1.9 % slower).
-constexpr uint64_t kWfWaves8MinJobs = uint64_t(64) << 20;
+// which parks the path state in LDS during traces (1080p workload, 1.37 M
+// jobs per frame, path kernel against 6 waves: 4 frames 4.5 % slower, 8
+// frames even, 12 / 16 / 20 frames 0.7 / 6.3 / 8.2 % faster, 256 frames 6.8 %
+// faster).
+constexpr uint64_t kWfWaves8MinJobs = uint64_t(12) << 20;
 __host__ __device__ __forceinline__ uint32_t wf_tree_waves(uint64_t jobs)
 {
     return jobs >= kWfWaves8MinJobs ? 8u : kWfWavesTree;
