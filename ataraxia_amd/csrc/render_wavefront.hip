@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) v
                 const BvhDev B = S->bvh;
                 BvhQuery q;
                 const float qd2 = (shadow && !(capture && S->dbg_shadow)) ? dist2 : -1.0f;
-                h = trace_bvh(B, flat, S->n_sph, ro, rd, qd2, q);
+                h = trace_bvh<true>(B, flat, S->n_sph, ro, rd, qd2, q);
                 if (q.exact)
                     atomicAdd(a.counters + kCtrExact, (unsigned long long)q.exact);
                 dt = q.tests;

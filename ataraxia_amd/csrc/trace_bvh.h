@@ -91,7 +91,8 @@ __device__ __forceinline__ float bvh_excess(float r, float D2)
 
 // Closest hit (dist2 < 0) or shadow query (dist2 >= 0) of ray (o, d).
 // `flat` reads sphere i of the scene in its original order (n_all spheres).
-template <class FlatSrc>
+// kPriority: raise the wave's priority in the node loop (below).
+template <bool kPriority = false, class FlatSrc>
 __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, uint32_t n_all, f3 o, f3 d,
                                          float dist2, BvhQuery& q)
 {
@@ -141,8 +142,21 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
     const ATX_GLOBAL float4* lsph = (const ATX_GLOBAL float4*)(uintptr_t)B.lsph;
     const ATX_GLOBAL int32_t* lidx = (const ATX_GLOBAL int32_t*)(uintptr_t)B.lidx;
     uint32_t i = (shadow && tlim * tlim < dist2) ? n_nodes : 0u;
+    // Wave priority (kPriority, for the persistent path kernel, whose waves
+    // do other work between traces): a wave in this loop issues before waves
+    // outside it (job hand-out, shading), and the node load of each trip
+    // before the rest of the loop's work, so that the loop's chain of
+    // dependent node fetches waits less for issue slots (1080p path kernel:
+    // 28.97 -> 27.12 ms per 256-frame launch, 3.15 -> 3.04 ms per 20-frame
+    // one).
+    if constexpr (kPriority)
+        __builtin_amdgcn_s_setprio(2);
     while (i < n_nodes) {
+        if constexpr (kPriority)
+            __builtin_amdgcn_s_setprio(3);
         const uint4 w = nodes[i];
+        if constexpr (kPriority)
+            __builtin_amdgcn_s_setprio(2);
         ++q.boxes;
         const float lx = (float)(w.x & 0xffffu), ly = (float)(w.x >> 16), lz = (float)(w.y & 0xffffu);
         const float hx = (float)(w.y >> 16), hy = (float)(w.z & 0xffffu), hz = (float)(w.z >> 16);
@@ -172,6 +186,8 @@ __device__ __forceinline__ Hit trace_bvh(const BvhDev& B, const FlatSrc& flat, u
         }
         i = next;
     }
+    if constexpr (kPriority)
+        __builtin_amdgcn_s_setprio(0);
 
     Hit h;
     if (f.amb != 0u) {
